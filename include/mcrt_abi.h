@@ -1,0 +1,265 @@
+/*
+ * mcrt_abi.h — C-ABI of libmcrt_hip.so, the MI355X (gfx950) render hot path.
+ *
+ * Part 1 restates the Pod buffer contract of the reference
+ * (src/gpu_layout.rs:78-234 bindings, the repr(C) structs listed per type).
+ * The byte layouts are identical, so buffers the reference host builds can be
+ * handed over unchanged.
+ *
+ * Part 2 declares the entry points.  Each one replaces a step of the
+ * reference's headless driver (src/headless.rs:112-386) or of its wgpu
+ * binding group, cited per function.
+ *
+ * Conventions
+ *  - Every function returns an int status: 0 = MCRT_OK, negative = error.
+ *    The per-context message is returned by mcrt_last_error().  Nothing in
+ *    the library aborts the process.
+ *  - Host pointers are read during the call only and are never retained.
+ *  - "_dev" variants take device pointers and a stream (hipStream_t passed as
+ *    void*; NULL = the null stream).  They enqueue work and return without
+ *    synchronising; the caller owns the memory and keeps it alive until the
+ *    stream has drained.
+ *  - One context per host thread.  Calls on one context are not thread-safe.
+ */
+#ifndef MCRT_ABI_H
+#define MCRT_ABI_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------------ */
+/* Part 1: Pod contract types (byte-identical to the reference)             */
+/* ------------------------------------------------------------------------ */
+
+/* src/scene.rs:22-29 — 32 B.  center_r.xyz = centre, .w = radius;
+ * mat_and_pad[0] = material index. */
+typedef struct GpuSphere {
+    float center_r[4];
+    uint32_t mat_and_pad[4];
+} GpuSphere;
+
+/* src/bvh.rs:54-68 — 48 B, pre-order flat BVH, root = 0.
+ * Internal (prim_count == 0): left child = index + 1, right = right_or_offset.
+ * Leaf (prim_count > 0): primitives [right_or_offset, +prim_count). */
+typedef struct GpuBvhNode {
+    float aabb_min[4];
+    float aabb_max[4];
+    uint32_t right_or_offset;
+    uint32_t prim_count;
+    uint32_t _pad[2];
+} GpuBvhNode;
+
+/* src/mesh.rs:24-34 — 48 B. */
+typedef struct GpuVertex {
+    float position[4];
+    float normal[4];
+    float uv[4];
+} GpuVertex;
+
+/* src/mesh.rs:39-46 — 16 B.  Bit 31 of mat_idx = back-face cull flag
+ * (src/scene.rs:709-718). */
+typedef struct GpuTriangle {
+    uint32_t v[3];
+    uint32_t mat_idx;
+} GpuTriangle;
+
+/* src/material.rs:35-44 — 48 B.  type_pad[0]: 0 lambertian, 1 metal,
+ * 2 dielectric, 3 emissive; type_pad[1]: albedo texture layer (0 = solid).
+ * Emissive: albedo_fuzz.xyz = colour, ior_pad[0] = strength. */
+typedef struct GpuMaterial {
+    uint32_t type_pad[4];
+    float albedo_fuzz[4];
+    float ior_pad[4];
+} GpuMaterial;
+
+#define MCRT_MAX_MATERIALS 64u /* src/material.rs:17 */
+
+/* src/material.rs:54-63 — 3088 B. */
+typedef struct GpuMaterialData {
+    uint32_t mat_count;
+    uint32_t n_emissive;
+    uint32_t _pad[2];
+    GpuMaterial materials[MCRT_MAX_MATERIALS];
+} GpuMaterialData;
+
+/* src/camera.rs:35-53 — 112 B. */
+typedef struct CameraUniform {
+    float origin[4];
+    float lower_left[4];
+    float horizontal[4];
+    float vertical[4];
+    float defocus_u[4];
+    float defocus_v[4];
+    uint32_t frame_count;
+    uint32_t _pad[3];
+} CameraUniform;
+
+/* src/gpu.rs:164-183 — 16 B.  Defaults: radius 3, sigma_s 2, sigma_d 0.5,
+ * normal_pow 8. */
+typedef struct DenoiseParams {
+    int32_t radius;
+    float sigma_s;
+    float sigma_d;
+    float normal_pow;
+} DenoiseParams;
+
+/* Constants of src/shaders/path_trace.wgsl:36-66. */
+#define MCRT_T_MIN 1e-4f
+#define MCRT_T_MAX 1e9f
+#define MCRT_MAX_FRAME_COUNT 65535u
+#define MCRT_MAX_BOUNCES 8u
+#define MCRT_MAT_LAMBERTIAN 0u
+#define MCRT_MAT_METAL 1u
+#define MCRT_MAT_DIELECTRIC 2u
+#define MCRT_MAT_EMISSIVE 3u
+#define MCRT_BACKFACE_CULL_FLAG 0x80000000u
+#define MCRT_ENV_MAP_WIDTH 1024u  /* src/texture.rs:33 */
+#define MCRT_ENV_MAP_HEIGHT 512u  /* src/texture.rs:36 */
+
+/* Primitive ids reported by the closest-hit query. */
+#define MCRT_ID_MISS 0xFFFFFFFFu
+#define MCRT_ID_SPHERE_BIT 0x80000000u /* id = bit | original sphere index */
+
+/* ------------------------------------------------------------------------ */
+/* Part 2: entry points                                                      */
+/* ------------------------------------------------------------------------ */
+
+enum {
+    MCRT_OK = 0,
+    MCRT_ERR_NO_DEVICE = -1,
+    MCRT_ERR_BAD_ARG = -2,
+    MCRT_ERR_OOM = -3,
+    MCRT_ERR_HIP = -4,
+    MCRT_ERR_LIMIT = -5,    /* e.g. more than 64 materials */
+    MCRT_ERR_NO_SCENE = -6, /* render/trace before mcrt_upload_scene */
+};
+
+typedef struct mcrt_ctx mcrt_ctx;
+
+/* Replaces the adapter/device request of src/headless.rs:135-148: binds a
+ * context to HIP device `device`.  Fails with MCRT_ERR_NO_DEVICE when the
+ * device does not exist (the reference: "no suitable GPU adapter"). */
+int mcrt_create(mcrt_ctx **out_ctx, int device);
+void mcrt_destroy(mcrt_ctx *ctx);
+
+/* Message of the last failed call on `ctx` ("" if none).  With ctx == NULL,
+ * the message of the last failed mcrt_create on this thread. */
+const char *mcrt_last_error(const mcrt_ctx *ctx);
+
+/* Replaces gpu_layout::build_scene_buffers (src/gpu_layout.rs:259-390) after
+ * the BVHs are built: uploads the scene in the reference layouts and derives
+ * the device-side traversal layout from them.
+ *  - spheres/sphere_bvh: output of build_bvh (src/bvh.rs:239), BVH-ordered.
+ *  - verts/tris/mesh_bvh: output of build_mesh_bvh (src/mesh.rs:82),
+ *    triangles in BVH leaf order.
+ *  - tri_orig_index: per ordered triangle, its index before the BVH build
+ *    (reported by the closest-hit query); NULL = identity.
+ *  - sphere_orig_index: likewise for spheres; NULL = identity.
+ *  - Empty sets may be passed as n = 0 or as the reference's stub entries
+ *    (STUB_SPHERE, zeroed node, src/gpu_layout.rs:264-312).
+ *  - env_rgba32f: env_w x env_h RGBA32F equirectangular map, row 0 = zenith
+ *    (src/texture.rs:158-220).  Required.
+ *  - albedo_rgba8: n_layers x size x size RGBA8 sRGB texture array
+ *    (src/gpu_layout.rs:435-477); NULL / 0 layers = solid colours only, as
+ *    the shader's empty-array guard (path_trace.wgsl:509-511). */
+int mcrt_upload_scene(mcrt_ctx *ctx,
+                      const GpuSphere *spheres, uint32_t n_spheres,
+                      const uint32_t *sphere_orig_index,
+                      const GpuBvhNode *sphere_bvh, uint32_t n_sphere_nodes,
+                      const GpuVertex *verts, uint32_t n_verts,
+                      const GpuTriangle *tris, uint32_t n_tris,
+                      const uint32_t *tri_orig_index,
+                      const GpuBvhNode *mesh_bvh, uint32_t n_mesh_nodes,
+                      const GpuMaterialData *materials,
+                      const float *env_rgba32f, uint32_t env_w, uint32_t env_h,
+                      const uint8_t *albedo_rgba8, uint32_t n_layers,
+                      uint32_t layer_size);
+
+/* Replaces the spp loop of src/headless.rs:249-290 (one cs_main dispatch per
+ * frame, path_trace.wgsl:626-662).  Frame i of the call uses
+ * camera->frame_count = first_frame + i for seeding and the running-mean
+ * weight 1/(min(f,65535)+1).  max_depth = bounce limit (the reference's
+ * MAX_BOUNCES = 8; 1..64 accepted).
+ *  - accum_in: W*H RGBA32F running mean after frames [0, first_frame), or
+ *    NULL for zeros (a fresh render, like the zero-initialised texture).
+ *  - out_accum: W*H RGBA32F running mean after the call (alpha = 1).
+ *  - out_gbuf: W*H RGBA32F G-buffer of the last frame (normal.xyz, depth),
+ *    sky = (0,0,0,1e9); may be NULL.
+ * Host-pointer variant; blocks until done. */
+int mcrt_render(mcrt_ctx *ctx, const CameraUniform *camera,
+                uint32_t width, uint32_t height, uint32_t spp,
+                uint32_t first_frame, uint32_t max_depth,
+                const float *accum_in, float *out_accum, float *out_gbuf);
+
+int mcrt_render_dev(mcrt_ctx *ctx, const CameraUniform *camera,
+                    uint32_t width, uint32_t height, uint32_t spp,
+                    uint32_t first_frame, uint32_t max_depth,
+                    const float *accum_in_dev, float *out_accum_dev,
+                    float *out_gbuf_dev, void *stream);
+
+/* Replaces cs_denoise (src/shaders/denoise.wgsl:52-120) as dispatched by
+ * src/gpu.rs:1150-1169.  accum/gbuf/out: W*H RGBA32F.  params == NULL uses
+ * the reference defaults (src/gpu.rs:177-183). */
+int mcrt_denoise_bilateral(mcrt_ctx *ctx, const DenoiseParams *params,
+                           uint32_t width, uint32_t height,
+                           const float *accum, const float *gbuf, float *out);
+
+int mcrt_denoise_bilateral_dev(mcrt_ctx *ctx, const DenoiseParams *params,
+                               uint32_t width, uint32_t height,
+                               const float *accum_dev, const float *gbuf_dev,
+                               float *out_dev, void *stream);
+
+/* Closest-hit query over the uploaded scene: world_hit
+ * (path_trace.wgsl:471-479) for n rays.  origins/dirs: n*3 floats (xyz
+ * interleaved); dirs need not be unit length.  Outputs: primitive id (original
+ * triangle index, MCRT_ID_SPHERE_BIT|sphere index, or MCRT_ID_MISS) and hit t
+ * (-1 on a miss).  Among primitives whose t is bitwise equal, the lowest
+ * original index wins; a sphere wins a tie against a triangle, as in
+ * world_hit.  Valid t lies in [tmin, tmax). */
+int mcrt_trace_closest(mcrt_ctx *ctx, const float *origins, const float *dirs,
+                       uint32_t n, float tmin, float tmax,
+                       uint32_t *out_id, float *out_t);
+
+int mcrt_trace_closest_dev(mcrt_ctx *ctx, const float *origins_dev,
+                           const float *dirs_dev, uint32_t n, float tmin,
+                           float tmax, uint32_t *out_id_dev, float *out_t_dev,
+                           void *stream);
+
+/* Binned-SAH BVH builders, src/mesh.rs:82-275 and src/bvh.rs:239-463 (8 bins,
+ * leaf <= 4, pre-order, left = i + 1).  Host-only; no device needed.
+ * Outputs are caller-allocated: out_nodes with room for 2*n - 1 nodes,
+ * out_ordered / out_orig_index with n entries.  *out_n_nodes receives the
+ * node count (0 for n = 0, as the reference).  Centroid ties in the split
+ * sort are broken by original index (the reference's sort_unstable_by leaves
+ * them unspecified). */
+int mcrt_build_mesh_bvh(const GpuVertex *verts, uint32_t n_verts,
+                        const GpuTriangle *tris, uint32_t n_tris,
+                        GpuBvhNode *out_nodes, uint32_t *out_n_nodes,
+                        GpuTriangle *out_ordered, uint32_t *out_orig_index);
+
+int mcrt_build_sphere_bvh(const GpuSphere *spheres, uint32_t n_spheres,
+                          GpuBvhNode *out_nodes, uint32_t *out_n_nodes,
+                          GpuSphere *out_ordered, uint32_t *out_orig_index);
+
+/* Kernel timing.  When enabled, every launch of the named kernels is
+ * bracketed by HIP events on its own stream; mcrt_kernel_stats returns the
+ * summed milliseconds and the launch count since the last reset.
+ * Names: "trace_closest", "extend", "shade", "generate", "accumulate",
+ * "bilateral". */
+int mcrt_set_profiling(mcrt_ctx *ctx, int enabled);
+int mcrt_kernel_stats(mcrt_ctx *ctx, const char *kernel, double *total_ms,
+                      uint64_t *launches);
+int mcrt_reset_stats(mcrt_ctx *ctx);
+
+/* Counters of the last mcrt_render*: path segments traced (rays) and
+ * samples produced.  Synchronises the context's last stream. */
+int mcrt_render_counters(mcrt_ctx *ctx, uint64_t *segments, uint64_t *samples);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MCRT_ABI_H */

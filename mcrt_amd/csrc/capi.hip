@@ -1,0 +1,742 @@
+// capi.hip — the extern "C" entry points of libmcrt_hip.so.
+//
+// Host logic of the drop-in boundary: context lifetime, validation and
+// conversion of the reference's buffer contract (src/gpu_layout.rs:259-390)
+// into the device layout of dev_common.h, and the stream-ordered launch
+// sequence that replaces the headless spp loop (src/headless.rs:249-290).
+// Nothing here computes radiance on the host: every render, trace and
+// denoise result comes from the HIP kernels.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/mcrt_abi.h"
+#include "kernels.h"
+
+using namespace mcrt;
+
+namespace {
+
+thread_local std::string g_create_error;
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+};
+
+struct Launch {
+    std::string name;
+    hipEvent_t start, stop;
+};
+
+}  // namespace
+
+struct mcrt_ctx {
+    int device = 0;
+    std::string err;
+    int num_cus = 0;
+    hipStream_t own_stream = nullptr;
+    hipStream_t last_stream = nullptr;
+    // scene
+    bool has_scene = false;
+    DevBuf mnodes, mtris, verts, tris_ref, snodes, sph, mats, env;
+    DevScene scene{};
+    uint32_t n_tris = 0, n_spheres = 0;
+    // work buffers
+    DevBuf A[2], B[2], C[2], H, samples;
+    size_t cap_paths = 0;
+    DevBuf counters;
+    size_t cap_counters = 0;
+    DevBuf host_accum, host_gbuf, host_out, host_accum_in;
+    DevBuf trace_o, trace_d, trace_id, trace_t, trace_work;
+    DevBuf spatial;
+    // last render bookkeeping
+    uint32_t last_batches = 0, last_depth = 0;
+    uint64_t last_samples = 0;
+    // profiling
+    bool prof = false;
+    std::vector<Launch> pending;
+    std::vector<hipEvent_t> pool;
+    std::map<std::string, double> total_ms;
+    std::map<std::string, uint64_t> launches;
+};
+
+namespace {
+
+int fail(mcrt_ctx *c, int code, const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    if (c) c->err = buf;
+    return code;
+}
+
+#define HIP_TRY(ctx, expr)                                                                \
+    do {                                                                                  \
+        hipError_t e_ = (expr);                                                           \
+        if (e_ != hipSuccess)                                                             \
+            return fail(ctx, e_ == hipErrorOutOfMemory ? MCRT_ERR_OOM : MCRT_ERR_HIP,      \
+                        "%s failed: %s", #expr, hipGetErrorString(e_));                   \
+    } while (0)
+
+int ensure(mcrt_ctx *c, DevBuf &b, size_t bytes) {
+    if (b.bytes >= bytes && b.p) return MCRT_OK;
+    if (b.p) {
+        (void)hipFree(b.p);
+        b.p = nullptr;
+        b.bytes = 0;
+    }
+    size_t want = std::max<size_t>(bytes, 256);
+    HIP_TRY(c, hipMalloc(&b.p, want));
+    b.bytes = want;
+    return MCRT_OK;
+}
+
+void release(DevBuf &b) {
+    if (b.p) (void)hipFree(b.p);
+    b.p = nullptr;
+    b.bytes = 0;
+}
+
+int upload(mcrt_ctx *c, DevBuf &b, const void *src, size_t bytes) {
+    int rc = ensure(c, b, bytes);
+    if (rc) return rc;
+    if (bytes) HIP_TRY(c, hipMemcpy(b.p, src, bytes, hipMemcpyHostToDevice));
+    return MCRT_OK;
+}
+
+int bind(mcrt_ctx *c) {
+    if (!c) return MCRT_ERR_BAD_ARG;
+    HIP_TRY(c, hipSetDevice(c->device));
+    return MCRT_OK;
+}
+
+// --- profiling -------------------------------------------------------------
+hipEvent_t take_event(mcrt_ctx *c) {
+    if (!c->pool.empty()) {
+        hipEvent_t e = c->pool.back();
+        c->pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+struct Timed {
+    mcrt_ctx *c;
+    const char *name;
+    hipStream_t s;
+    hipEvent_t a = nullptr, b = nullptr;
+    Timed(mcrt_ctx *c_, const char *n, hipStream_t s_) : c(c_), name(n), s(s_) {
+        if (c->prof) {
+            a = take_event(c);
+            b = take_event(c);
+            if (a) (void)hipEventRecord(a, s);
+        }
+    }
+    ~Timed() {
+        if (a && b) {
+            (void)hipEventRecord(b, s);
+            c->pending.push_back(Launch{name, a, b});
+        }
+    }
+};
+
+int drain_profile(mcrt_ctx *c) {
+    for (Launch &l : c->pending) {
+        HIP_TRY(c, hipEventSynchronize(l.stop));
+        float ms = 0.0f;
+        HIP_TRY(c, hipEventElapsedTime(&ms, l.start, l.stop));
+        c->total_ms[l.name] += ms;
+        c->launches[l.name] += 1;
+        c->pool.push_back(l.start);
+        c->pool.push_back(l.stop);
+    }
+    c->pending.clear();
+    return MCRT_OK;
+}
+
+// --- scene conversion -------------------------------------------------------
+struct TreeOut {
+    std::vector<float4> nodes;  // 4 per internal node
+    uint32_t root = kNone;
+    float lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+    uint32_t depth = 1;
+    std::vector<uint8_t> last;  // per primitive: 1 if last in its leaf
+};
+
+bool is_stub_tree(const GpuBvhNode *nodes, uint32_t n) {
+    return n == 0 || (n == 1 && nodes[0].prim_count == 0);
+}
+
+// Validates a reference pre-order tree and converts it to child pairs.
+int convert_tree(mcrt_ctx *c, const char *what, const GpuBvhNode *nodes, uint32_t n,
+                 uint32_t n_prims, TreeOut &out) {
+    out.last.assign(n_prims, 0);
+    if (is_stub_tree(nodes, n)) return MCRT_OK;
+    if (!nodes) return fail(c, MCRT_ERR_BAD_ARG, "%s BVH: null node array", what);
+    std::vector<uint32_t> dev_index(n, kNone);
+    std::vector<uint8_t> covered(n_prims, 0);
+    uint32_t n_internal = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        const GpuBvhNode &nd = nodes[i];
+        if (nd.prim_count == 0) {
+            if (i + 1 >= n || nd.right_or_offset <= i + 1 || nd.right_or_offset >= n)
+                return fail(c, MCRT_ERR_BAD_ARG,
+                            "%s BVH: internal node %u has children (%u, %u) outside the pre-order "
+                            "layout of %u nodes",
+                            what, i, i + 1, nd.right_or_offset, n);
+            dev_index[i] = n_internal++;
+        } else {
+            uint64_t end = (uint64_t)nd.right_or_offset + nd.prim_count;
+            if (end > n_prims)
+                return fail(c, MCRT_ERR_BAD_ARG, "%s BVH: leaf %u covers [%u, %llu) beyond %u primitives",
+                            what, i, nd.right_or_offset, (unsigned long long)end, n_prims);
+            for (uint64_t k = nd.right_or_offset; k < end; ++k) {
+                if (covered[k])
+                    return fail(c, MCRT_ERR_BAD_ARG, "%s BVH: primitive %llu is in two leaves", what,
+                                (unsigned long long)k);
+                covered[k] = 1;
+            }
+            out.last[end - 1] = 1;
+        }
+    }
+    auto ref = [&](uint32_t i) -> uint32_t {
+        return nodes[i].prim_count == 0 ? dev_index[i] : (kLeafBit | nodes[i].right_or_offset);
+    };
+    out.nodes.assign((size_t)std::max<uint32_t>(n_internal, 1) * 4, make_float4(0, 0, 0, 0));
+    for (uint32_t i = 0; i < n; ++i) {
+        if (nodes[i].prim_count != 0) continue;
+        const GpuBvhNode &a = nodes[i + 1];
+        const GpuBvhNode &b = nodes[nodes[i].right_or_offset];
+        float4 *q = &out.nodes[4 * (size_t)dev_index[i]];
+        q[0] = make_float4(a.aabb_min[0], a.aabb_max[0], a.aabb_min[1], a.aabb_max[1]);
+        q[1] = make_float4(b.aabb_min[0], b.aabb_max[0], b.aabb_min[1], b.aabb_max[1]);
+        q[2] = make_float4(a.aabb_min[2], a.aabb_max[2], b.aabb_min[2], b.aabb_max[2]);
+        uint32_t r0 = ref(i + 1), r1 = ref(nodes[i].right_or_offset);
+        float f0, f1;
+        std::memcpy(&f0, &r0, 4);
+        std::memcpy(&f1, &r1, 4);
+        q[3] = make_float4(f0, f1, 0.0f, 0.0f);
+    }
+    out.root = ref(0);
+    for (int k = 0; k < 3; ++k) {
+        out.lo[k] = nodes[0].aabb_min[k];
+        out.hi[k] = nodes[0].aabb_max[k];
+    }
+    // Largest number of internal nodes on a root-to-leaf path: the traversal
+    // pushes at most one far child per internal node on its current path.
+    uint32_t maxd = 0;
+    std::vector<std::pair<uint32_t, uint32_t>> st;
+    st.push_back({0u, 0u});
+    while (!st.empty()) {
+        auto [i, d] = st.back();
+        st.pop_back();
+        if (nodes[i].prim_count == 0) {
+            st.push_back({i + 1, d + 1});
+            st.push_back({nodes[i].right_or_offset, d + 1});
+        } else {
+            maxd = std::max(maxd, d);
+        }
+    }
+    out.depth = std::max<uint32_t>(maxd, 1);
+    return MCRT_OK;
+}
+
+inline float ubits(uint32_t u) {
+    float f;
+    std::memcpy(&f, &u, 4);
+    return f;
+}
+
+// Frames per batch: keep a batch at or below 2^24 paths.
+uint32_t frames_per_batch(uint32_t wh, uint32_t spp) {
+    uint64_t cap = (1ull << 24) / std::max<uint32_t>(wh, 1);
+    return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(cap, spp));
+}
+
+int grid_for(mcrt_ctx *c, int which, uint32_t *grid) {
+    int per_cu = 0;
+    hipError_t e = occupancy(which, c->scene.stack_depth, &per_cu);
+    if (e != hipSuccess || per_cu < 1) per_cu = 1;
+    *grid = (uint32_t)(c->num_cus * per_cu);
+    return MCRT_OK;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+extern "C" int mcrt_create(mcrt_ctx **out_ctx, int device) {
+    if (!out_ctx) return MCRT_ERR_BAD_ARG;
+    *out_ctx = nullptr;
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || n <= 0) {
+        g_create_error = "no suitable GPU adapter: no HIP device";
+        return MCRT_ERR_NO_DEVICE;
+    }
+    if (device < 0 || device >= n) {
+        g_create_error = "no suitable GPU adapter: device index out of range";
+        return MCRT_ERR_NO_DEVICE;
+    }
+    mcrt_ctx *c = new mcrt_ctx();
+    c->device = device;
+    hipDeviceProp_t prop;
+    if (hipSetDevice(device) != hipSuccess || hipGetDeviceProperties(&prop, device) != hipSuccess ||
+        hipStreamCreate(&c->own_stream) != hipSuccess) {
+        g_create_error = "no suitable GPU adapter: HIP initialisation failed";
+        delete c;
+        return MCRT_ERR_NO_DEVICE;
+    }
+    c->num_cus = prop.multiProcessorCount;
+    *out_ctx = c;
+    return MCRT_OK;
+}
+
+extern "C" void mcrt_destroy(mcrt_ctx *c) {
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->last_stream);
+    (void)hipStreamSynchronize(c->own_stream);
+    for (DevBuf *b : {&c->mnodes, &c->mtris, &c->verts, &c->tris_ref, &c->snodes, &c->sph, &c->mats,
+                      &c->env, &c->A[0], &c->A[1], &c->B[0], &c->B[1], &c->C[0], &c->C[1], &c->H,
+                      &c->samples, &c->counters, &c->host_accum, &c->host_gbuf, &c->host_out,
+                      &c->host_accum_in, &c->trace_o, &c->trace_d, &c->trace_id, &c->trace_t,
+                      &c->trace_work, &c->spatial})
+        release(*b);
+    for (Launch &l : c->pending) {
+        (void)hipEventDestroy(l.start);
+        (void)hipEventDestroy(l.stop);
+    }
+    for (hipEvent_t e : c->pool) (void)hipEventDestroy(e);
+    if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
+    delete c;
+}
+
+extern "C" const char *mcrt_last_error(const mcrt_ctx *c) {
+    return c ? c->err.c_str() : g_create_error.c_str();
+}
+
+extern "C" int mcrt_upload_scene(mcrt_ctx *c, const GpuSphere *spheres, uint32_t n_spheres,
+                                 const uint32_t *sphere_orig_index, const GpuBvhNode *sphere_bvh,
+                                 uint32_t n_sphere_nodes, const GpuVertex *verts, uint32_t n_verts,
+                                 const GpuTriangle *tris, uint32_t n_tris,
+                                 const uint32_t *tri_orig_index, const GpuBvhNode *mesh_bvh,
+                                 uint32_t n_mesh_nodes, const GpuMaterialData *materials,
+                                 const float *env_rgba32f, uint32_t env_w, uint32_t env_h,
+                                 const uint8_t *albedo_rgba8, uint32_t n_layers,
+                                 uint32_t layer_size) {
+    int rc = bind(c);
+    if (rc) return rc;
+    c->has_scene = false;
+    if (!materials) return fail(c, MCRT_ERR_BAD_ARG, "materials must not be null");
+    if (materials->mat_count > MCRT_MAX_MATERIALS)
+        return fail(c, MCRT_ERR_LIMIT, "scene uses %u materials but limit is %u", materials->mat_count,
+                    MCRT_MAX_MATERIALS);
+    if (!env_rgba32f || env_w == 0 || env_h == 0)
+        return fail(c, MCRT_ERR_BAD_ARG, "environment map is required (got %ux%u)", env_w, env_h);
+    if ((n_spheres && !spheres) || (n_tris && (!tris || !verts)))
+        return fail(c, MCRT_ERR_BAD_ARG, "null primitive array with a non-zero count");
+    if (n_layers > 0 && albedo_rgba8) {
+        for (uint32_t m = 0; m < MCRT_MAX_MATERIALS; ++m) {
+            const GpuMaterial &g = materials->materials[m];
+            if (g.type_pad[1] != 0 && g.type_pad[0] != MCRT_MAT_EMISSIVE && m < materials->mat_count)
+                return fail(c, MCRT_ERR_BAD_ARG,
+                            "material %u samples albedo texture layer %u; texture sampling is not "
+                            "implemented in this build",
+                            m, g.type_pad[1]);
+        }
+    }
+    (void)layer_size;
+    for (uint32_t i = 0; i < n_tris; ++i)
+        for (int k = 0; k < 3; ++k)
+            if (tris[i].v[k] >= n_verts)
+                return fail(c, MCRT_ERR_BAD_ARG, "triangle %u references vertex %u of %u", i,
+                            tris[i].v[k], n_verts);
+
+    TreeOut mt, st;
+    rc = convert_tree(c, "mesh", mesh_bvh, n_mesh_nodes, n_tris, mt);
+    if (rc) return rc;
+    rc = convert_tree(c, "sphere", sphere_bvh, n_sphere_nodes, n_spheres, st);
+    if (rc) return rc;
+
+    // Triangles in leaf order: v0, e1 = v1 - v0, e2 = v2 - v0.
+    std::vector<float4> dt((size_t)std::max<uint32_t>(n_tris, 1) * 3, make_float4(0, 0, 0, 0));
+    for (uint32_t i = 0; i < n_tris; ++i) {
+        const float *p0 = verts[tris[i].v[0]].position;
+        const float *p1 = verts[tris[i].v[1]].position;
+        const float *p2 = verts[tris[i].v[2]].position;
+        uint32_t flags = (mt.last[i] ? kPrimLast : 0u) | ((tris[i].mat_idx >> 31) ? kPrimCull : 0u);
+        uint32_t orig = tri_orig_index ? tri_orig_index[i] : i;
+        dt[3 * (size_t)i + 0] = make_float4(p0[0], p0[1], p0[2], ubits(orig));
+        dt[3 * (size_t)i + 1] =
+            make_float4(p1[0] - p0[0], p1[1] - p0[1], p1[2] - p0[2], ubits(flags));
+        dt[3 * (size_t)i + 2] = make_float4(p2[0] - p0[0], p2[1] - p0[1], p2[2] - p0[2], ubits(i));
+    }
+    std::vector<float4> ds((size_t)std::max<uint32_t>(n_spheres, 1) * 2, make_float4(0, 0, 0, 0));
+    for (uint32_t i = 0; i < n_spheres; ++i) {
+        const GpuSphere &s = spheres[i];
+        uint32_t orig = sphere_orig_index ? sphere_orig_index[i] : i;
+        ds[2 * (size_t)i] = make_float4(s.center_r[0], s.center_r[1], s.center_r[2], s.center_r[3]);
+        ds[2 * (size_t)i + 1] = make_float4(ubits(s.mat_and_pad[0]), ubits(orig),
+                                            ubits(st.last[i] ? kPrimLast : 0u), 0.0f);
+    }
+    std::vector<float4> denv((size_t)env_w * env_h);
+    std::memcpy(denv.data(), env_rgba32f, denv.size() * sizeof(float4));
+
+    if ((rc = upload(c, c->mnodes, mt.nodes.data(), mt.nodes.size() * sizeof(float4)))) return rc;
+    if ((rc = upload(c, c->mtris, dt.data(), dt.size() * sizeof(float4)))) return rc;
+    if ((rc = upload(c, c->verts, verts ? (const void *)verts : (const void *)dt.data(),
+                     verts ? (size_t)n_verts * sizeof(GpuVertex) : 0)))
+        return rc;
+    if ((rc = upload(c, c->tris_ref, tris ? (const void *)tris : (const void *)dt.data(),
+                     tris ? (size_t)n_tris * sizeof(GpuTriangle) : 0)))
+        return rc;
+    if ((rc = upload(c, c->snodes, st.nodes.data(), st.nodes.size() * sizeof(float4)))) return rc;
+    if ((rc = upload(c, c->sph, ds.data(), ds.size() * sizeof(float4)))) return rc;
+    if ((rc = upload(c, c->mats, materials->materials, sizeof(materials->materials)))) return rc;
+    if ((rc = upload(c, c->env, denv.data(), denv.size() * sizeof(float4)))) return rc;
+
+    DevScene &d = c->scene;
+    d = DevScene{};
+    d.mnodes = (const float4 *)c->mnodes.p;
+    d.mtris = (const float4 *)c->mtris.p;
+    d.verts = (const GpuVertex *)c->verts.p;
+    d.tris_ref = (const uint4 *)c->tris_ref.p;
+    d.mroot = n_tris ? mt.root : kNone;
+    std::memcpy(d.mroot_lo, mt.lo, sizeof mt.lo);
+    std::memcpy(d.mroot_hi, mt.hi, sizeof mt.hi);
+    d.snodes = (const float4 *)c->snodes.p;
+    d.sph = (const float4 *)c->sph.p;
+    d.sroot = n_spheres ? st.root : kNone;
+    std::memcpy(d.sroot_lo, st.lo, sizeof st.lo);
+    std::memcpy(d.sroot_hi, st.hi, sizeof st.hi);
+    d.mats = (const GpuMaterial *)c->mats.p;
+    d.env = (const float4 *)c->env.p;
+    d.env_w = env_w;
+    d.env_h = env_h;
+    d.stack_depth = std::max(mt.depth, st.depth);
+    if (d.stack_depth > 96)
+        return fail(c, MCRT_ERR_LIMIT, "BVH depth %u exceeds the traversal stack limit of 96",
+                    d.stack_depth);
+    c->n_tris = n_tris;
+    c->n_spheres = n_spheres;
+    c->has_scene = true;
+    return MCRT_OK;
+}
+
+extern "C" int mcrt_render_dev(mcrt_ctx *c, const CameraUniform *cam, uint32_t width,
+                               uint32_t height, uint32_t spp, uint32_t first_frame,
+                               uint32_t max_depth, const float *accum_in_dev, float *out_accum_dev,
+                               float *out_gbuf_dev, void *stream) {
+    int rc = bind(c);
+    if (rc) return rc;
+    if (!c->has_scene) return fail(c, MCRT_ERR_NO_SCENE, "render called before mcrt_upload_scene");
+    if (!cam || !out_accum_dev) return fail(c, MCRT_ERR_BAD_ARG, "camera and out_accum are required");
+    if (width == 0 || height == 0 || spp == 0)
+        return fail(c, MCRT_ERR_BAD_ARG, "width, height and spp must be positive");
+    if (max_depth < 1 || max_depth > 64)
+        return fail(c, MCRT_ERR_BAD_ARG, "max_depth must be in [1, 64] (got %u)", max_depth);
+    uint64_t wh64 = (uint64_t)width * height;
+    if (wh64 > (1ull << 26)) return fail(c, MCRT_ERR_LIMIT, "image of %llu pixels is too large",
+                                         (unsigned long long)wh64);
+    hipStream_t s = (hipStream_t)stream;
+    c->last_stream = s;
+    const uint32_t wh = (uint32_t)wh64;
+    const uint32_t F = frames_per_batch(wh, spp);
+    const size_t cap = (size_t)wh * F;
+    if (cap > c->cap_paths) {
+        for (DevBuf *b : {&c->A[0], &c->A[1], &c->B[0], &c->B[1], &c->C[0], &c->C[1], &c->H, &c->samples})
+            release(*b);
+        for (DevBuf *b : {&c->A[0], &c->A[1], &c->B[0], &c->B[1], &c->C[0], &c->C[1], &c->H, &c->samples})
+            if ((rc = ensure(c, *b, cap * sizeof(float4)))) return rc;
+        c->cap_paths = cap;
+    }
+    const uint32_t n_batches = (spp + F - 1) / F;
+    const uint32_t per_batch = 3u * (max_depth + 1u);
+    const size_t n_counters = (size_t)per_batch * n_batches;
+    if ((rc = ensure(c, c->counters, n_counters * sizeof(uint32_t)))) return rc;
+    uint32_t *counters = (uint32_t *)c->counters.p;
+    HIP_TRY(c, launch_fill_u32(counters, (uint32_t)n_counters, 0u, s));
+
+    uint32_t g_ext = 0, g_shade = 0;
+    grid_for(c, 0, &g_ext);
+    grid_for(c, 1, &g_shade);
+
+    const uint32_t last_frame = first_frame + spp - 1u;
+    for (uint32_t bi = 0; bi < n_batches; ++bi) {
+        const uint32_t f0 = first_frame + bi * F;
+        const uint32_t nf = std::min<uint32_t>(F, spp - bi * F);
+        const uint32_t np = wh * nf;
+        uint32_t *cnt = counters + (size_t)bi * per_batch;  // [max_depth+1] path counts
+        uint32_t *wext = cnt + (max_depth + 1u);
+        uint32_t *wsh = wext + (max_depth + 1u);
+        HIP_TRY(c, launch_fill_u32(cnt, 1u, np, s));
+        GenArgs g{};
+        g.cam = *cam;
+        g.width = width;
+        g.height = height;
+        g.wh = wh;
+        g.frame0 = f0;
+        g.n_paths = np;
+        g.A = (float4 *)c->A[0].p;
+        g.B = (float4 *)c->B[0].p;
+        g.C = (float4 *)c->C[0].p;
+        {
+            Timed t(c, "generate", s);
+            HIP_TRY(c, launch_generate(g, s));
+        }
+        for (uint32_t b = 0; b < max_depth; ++b) {
+            const int cur = b & 1, nxt = cur ^ 1;
+            ExtArgs e{};
+            e.scene = c->scene;
+            e.A = (const float4 *)c->A[cur].p;
+            e.B = (const float4 *)c->B[cur].p;
+            e.H = (float4 *)c->H.p;
+            e.count = cnt + b;
+            e.work = wext + b;
+            {
+                Timed t(c, "extend", s);
+                HIP_TRY(c, launch_extend(e, g_ext, s));
+            }
+            ShadeArgs sh{};
+            sh.scene = c->scene;
+            sh.A = (const float4 *)c->A[cur].p;
+            sh.B = (const float4 *)c->B[cur].p;
+            sh.C = (const float4 *)c->C[cur].p;
+            sh.H = (const float4 *)c->H.p;
+            sh.A_out = (float4 *)c->A[nxt].p;
+            sh.B_out = (float4 *)c->B[nxt].p;
+            sh.C_out = (float4 *)c->C[nxt].p;
+            sh.count_in = cnt + b;
+            sh.count_out = cnt + b + 1;
+            sh.work = wsh + b;
+            sh.samples = (float4 *)c->samples.p;
+            const bool has_last = last_frame >= f0 && last_frame < f0 + nf;
+            sh.gbuf = (has_last && b == 0) ? (float4 *)out_gbuf_dev : nullptr;
+            sh.gbuf_frame_index = last_frame - f0;
+            sh.wh = wh;
+            sh.max_depth = max_depth;
+            {
+                Timed t(c, "shade", s);
+                HIP_TRY(c, launch_shade(sh, g_shade, s));
+            }
+        }
+        AccArgs acc{};
+        acc.samples = (const float4 *)c->samples.p;
+        acc.accum_in = bi == 0 ? (const float4 *)accum_in_dev : (const float4 *)out_accum_dev;
+        acc.accum_out = (float4 *)out_accum_dev;
+        acc.wh = wh;
+        acc.n_frames = nf;
+        acc.frame0 = f0;
+        {
+            Timed t(c, "accumulate", s);
+            HIP_TRY(c, launch_accumulate(acc, s));
+        }
+    }
+    c->last_batches = n_batches;
+    c->last_depth = max_depth;
+    c->last_samples = (uint64_t)wh * spp;
+    return MCRT_OK;
+}
+
+extern "C" int mcrt_render(mcrt_ctx *c, const CameraUniform *cam, uint32_t width, uint32_t height,
+                           uint32_t spp, uint32_t first_frame, uint32_t max_depth,
+                           const float *accum_in, float *out_accum, float *out_gbuf) {
+    int rc = bind(c);
+    if (rc) return rc;
+    if (!out_accum) return fail(c, MCRT_ERR_BAD_ARG, "out_accum is required");
+    if (width == 0 || height == 0) return fail(c, MCRT_ERR_BAD_ARG, "width and height must be positive");
+    size_t bytes = (size_t)width * height * 16;
+    if ((rc = ensure(c, c->host_accum, bytes))) return rc;
+    if ((rc = ensure(c, c->host_gbuf, bytes))) return rc;
+    hipStream_t s = c->own_stream;
+    const float *din = nullptr;
+    if (accum_in) {
+        if ((rc = ensure(c, c->host_accum_in, bytes))) return rc;
+        HIP_TRY(c, hipMemcpyAsync(c->host_accum_in.p, accum_in, bytes, hipMemcpyHostToDevice, s));
+        din = (const float *)c->host_accum_in.p;
+    }
+    rc = mcrt_render_dev(c, cam, width, height, spp, first_frame, max_depth, din,
+                         (float *)c->host_accum.p, out_gbuf ? (float *)c->host_gbuf.p : nullptr, s);
+    if (rc) return rc;
+    HIP_TRY(c, hipMemcpyAsync(out_accum, c->host_accum.p, bytes, hipMemcpyDeviceToHost, s));
+    if (out_gbuf)
+        HIP_TRY(c, hipMemcpyAsync(out_gbuf, c->host_gbuf.p, bytes, hipMemcpyDeviceToHost, s));
+    HIP_TRY(c, hipStreamSynchronize(s));
+    return MCRT_OK;
+}
+
+extern "C" int mcrt_render_counters(mcrt_ctx *c, uint64_t *segments, uint64_t *samples) {
+    int rc = bind(c);
+    if (rc) return rc;
+    if (c->last_batches == 0) return fail(c, MCRT_ERR_NO_SCENE, "no render has run on this context");
+    HIP_TRY(c, hipStreamSynchronize(c->last_stream));
+    const uint32_t per_batch = 3u * (c->last_depth + 1u);
+    std::vector<uint32_t> h((size_t)per_batch * c->last_batches);
+    HIP_TRY(c, hipMemcpy(h.data(), c->counters.p, h.size() * 4, hipMemcpyDeviceToHost));
+    uint64_t seg = 0;
+    for (uint32_t bi = 0; bi < c->last_batches; ++bi)
+        for (uint32_t b = 0; b < c->last_depth; ++b) seg += h[(size_t)bi * per_batch + b];
+    if (segments) *segments = seg;
+    if (samples) *samples = c->last_samples;
+    return MCRT_OK;
+}
+
+extern "C" int mcrt_trace_closest_dev(mcrt_ctx *c, const float *origins_dev, const float *dirs_dev,
+                                      uint32_t n, float tmin, float tmax, uint32_t *out_id_dev,
+                                      float *out_t_dev, void *stream) {
+    int rc = bind(c);
+    if (rc) return rc;
+    if (!c->has_scene) return fail(c, MCRT_ERR_NO_SCENE, "trace called before mcrt_upload_scene");
+    if (n == 0) return MCRT_OK;
+    if (!origins_dev || !dirs_dev || !out_id_dev || !out_t_dev)
+        return fail(c, MCRT_ERR_BAD_ARG, "null ray or output pointer");
+    if (n > 0xFFFFFFFFu - 64u) return fail(c, MCRT_ERR_LIMIT, "too many rays");
+    hipStream_t s = (hipStream_t)stream;
+    c->last_stream = s;
+    if ((rc = ensure(c, c->trace_work, 4))) return rc;
+    HIP_TRY(c, launch_fill_u32((uint32_t *)c->trace_work.p, 1u, 0u, s));
+    TraceArgs a{};
+    a.scene = c->scene;
+    a.origins = origins_dev;
+    a.dirs = dirs_dev;
+    a.n = n;
+    a.tmin = tmin;
+    a.tmax = tmax;
+    a.out_id = out_id_dev;
+    a.out_t = out_t_dev;
+    a.work = (uint32_t *)c->trace_work.p;
+    uint32_t grid = 0;
+    grid_for(c, 2, &grid);
+    uint32_t need = (n + 63u) / 64u;  // waves of work
+    uint32_t waves_per_block = (uint32_t)trace_block() / 64u;
+    grid = std::min(grid, (need + waves_per_block - 1) / waves_per_block);
+    {
+        Timed t(c, "trace_closest", s);
+        HIP_TRY(c, launch_trace(a, std::max<uint32_t>(grid, 1), s));
+    }
+    return MCRT_OK;
+}
+
+extern "C" int mcrt_trace_closest(mcrt_ctx *c, const float *origins, const float *dirs, uint32_t n,
+                                  float tmin, float tmax, uint32_t *out_id, float *out_t) {
+    int rc = bind(c);
+    if (rc) return rc;
+    if (n == 0) return MCRT_OK;
+    if (!origins || !dirs || !out_id || !out_t)
+        return fail(c, MCRT_ERR_BAD_ARG, "null ray or output pointer");
+    size_t b3 = (size_t)n * 12, b1 = (size_t)n * 4;
+    if ((rc = ensure(c, c->trace_o, b3)) || (rc = ensure(c, c->trace_d, b3)) ||
+        (rc = ensure(c, c->trace_id, b1)) || (rc = ensure(c, c->trace_t, b1)))
+        return rc;
+    hipStream_t s = c->own_stream;
+    HIP_TRY(c, hipMemcpyAsync(c->trace_o.p, origins, b3, hipMemcpyHostToDevice, s));
+    HIP_TRY(c, hipMemcpyAsync(c->trace_d.p, dirs, b3, hipMemcpyHostToDevice, s));
+    rc = mcrt_trace_closest_dev(c, (const float *)c->trace_o.p, (const float *)c->trace_d.p, n, tmin,
+                                tmax, (uint32_t *)c->trace_id.p, (float *)c->trace_t.p, s);
+    if (rc) return rc;
+    HIP_TRY(c, hipMemcpyAsync(out_id, c->trace_id.p, b1, hipMemcpyDeviceToHost, s));
+    HIP_TRY(c, hipMemcpyAsync(out_t, c->trace_t.p, b1, hipMemcpyDeviceToHost, s));
+    HIP_TRY(c, hipStreamSynchronize(s));
+    return MCRT_OK;
+}
+
+extern "C" int mcrt_denoise_bilateral_dev(mcrt_ctx *c, const DenoiseParams *params, uint32_t width,
+                                          uint32_t height, const float *accum_dev,
+                                          const float *gbuf_dev, float *out_dev, void *stream) {
+    int rc = bind(c);
+    if (rc) return rc;
+    DenoiseParams p = params ? *params : DenoiseParams{3, 2.0f, 0.5f, 8.0f};
+    if (width == 0 || height == 0) return MCRT_OK;
+    if (!accum_dev || !gbuf_dev || !out_dev) return fail(c, MCRT_ERR_BAD_ARG, "null image pointer");
+    if (p.radius < 0 || p.radius > 8)
+        return fail(c, MCRT_ERR_LIMIT, "denoise radius %d outside the supported range [0, 8]", p.radius);
+    if (width > 0x7FFFFFFFu || height > 0x7FFFFFFFu) return fail(c, MCRT_ERR_LIMIT, "image too large");
+    hipStream_t s = (hipStream_t)stream;
+    c->last_stream = s;
+    // Spatial weights exp(-dist_sq * inv2ss), denoise.wgsl:65 and :88-89.
+    const int R = p.radius, span = 2 * R + 1;
+    const float inv2ss = 1.0f / (2.0f * p.sigma_s * p.sigma_s);
+    std::vector<float> lut((size_t)span * span);
+    for (int dy = -R, k = 0; dy <= R; ++dy)
+        for (int dx = -R; dx <= R; ++dx, ++k) lut[k] = std::exp(-(float)(dx * dx + dy * dy) * inv2ss);
+    if ((rc = ensure(c, c->spatial, lut.size() * 4))) return rc;
+    HIP_TRY(c, hipMemcpyAsync(c->spatial.p, lut.data(), lut.size() * 4, hipMemcpyHostToDevice, s));
+    BilateralArgs a{};
+    a.accum = (const float4 *)accum_dev;
+    a.gbuf = (const float4 *)gbuf_dev;
+    a.out = (float4 *)out_dev;
+    a.width = (int32_t)width;
+    a.height = (int32_t)height;
+    a.radius = R;
+    a.inv2ds = 1.0f / (2.0f * p.sigma_d * p.sigma_d);
+    a.normal_pow = p.normal_pow;
+    a.spatial = (const float *)c->spatial.p;
+    {
+        Timed t(c, "bilateral", s);
+        HIP_TRY(c, launch_bilateral(a, s));
+    }
+    // The LUT upload is stream-ordered before the kernel; keep the host
+    // vector alive until the copy has been consumed.
+    HIP_TRY(c, hipStreamSynchronize(s));
+    return MCRT_OK;
+}
+
+extern "C" int mcrt_denoise_bilateral(mcrt_ctx *c, const DenoiseParams *params, uint32_t width,
+                                      uint32_t height, const float *accum, const float *gbuf,
+                                      float *out) {
+    int rc = bind(c);
+    if (rc) return rc;
+    if (width == 0 || height == 0) return MCRT_OK;
+    if (!accum || !gbuf || !out) return fail(c, MCRT_ERR_BAD_ARG, "null image pointer");
+    size_t bytes = (size_t)width * height * 16;
+    if ((rc = ensure(c, c->host_accum, bytes)) || (rc = ensure(c, c->host_gbuf, bytes)) ||
+        (rc = ensure(c, c->host_out, bytes)))
+        return rc;
+    hipStream_t s = c->own_stream;
+    HIP_TRY(c, hipMemcpyAsync(c->host_accum.p, accum, bytes, hipMemcpyHostToDevice, s));
+    HIP_TRY(c, hipMemcpyAsync(c->host_gbuf.p, gbuf, bytes, hipMemcpyHostToDevice, s));
+    rc = mcrt_denoise_bilateral_dev(c, params, width, height, (const float *)c->host_accum.p,
+                                    (const float *)c->host_gbuf.p, (float *)c->host_out.p, s);
+    if (rc) return rc;
+    HIP_TRY(c, hipMemcpyAsync(out, c->host_out.p, bytes, hipMemcpyDeviceToHost, s));
+    HIP_TRY(c, hipStreamSynchronize(s));
+    return MCRT_OK;
+}
+
+extern "C" int mcrt_set_profiling(mcrt_ctx *c, int enabled) {
+    if (!c) return MCRT_ERR_BAD_ARG;
+    c->prof = enabled != 0;
+    return MCRT_OK;
+}
+
+extern "C" int mcrt_kernel_stats(mcrt_ctx *c, const char *kernel, double *total_ms,
+                                 uint64_t *launches) {
+    int rc = bind(c);
+    if (rc) return rc;
+    if (!kernel) return fail(c, MCRT_ERR_BAD_ARG, "kernel name is required");
+    if ((rc = drain_profile(c))) return rc;
+    if (total_ms) *total_ms = c->total_ms.count(kernel) ? c->total_ms[kernel] : 0.0;
+    if (launches) *launches = c->launches.count(kernel) ? c->launches[kernel] : 0;
+    return MCRT_OK;
+}
+
+extern "C" int mcrt_reset_stats(mcrt_ctx *c) {
+    int rc = bind(c);
+    if (rc) return rc;
+    if ((rc = drain_profile(c))) return rc;
+    c->total_ms.clear();
+    c->launches.clear();
+    return MCRT_OK;
+}
