@@ -9,7 +9,11 @@ Output: one JSON line with Msamples/s, the roofline of the dominant kernel
 (``extend``, the closest-hit pass) measured with HIP events on its launch
 stream, and the CPU oracle timed on a bounded sample of the same workload.
 
-    python bench.py [--steps K] [--warmup W] [--workload c2|c1]
+    python bench.py [--steps K] [--warmup W] [--workload c3|c2|c1]
+
+Default workload: the voxel terrain at 1920 x 1080, 16 spp, depth 6, with the
+bilateral denoiser in every step (the 1080p configuration of the metric).
+The 16M-ray closest-hit microbenchmark runs on the displaced-icosphere scene.
 """
 from __future__ import annotations
 
@@ -31,7 +35,7 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=5)
     p.add_argument("--warmup", type=int, default=2)
-    p.add_argument("--workload", default="c2")
+    p.add_argument("--workload", default="c3")
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--no-roofline", action="store_true")
     p.add_argument("--no-microbench", action="store_true")
@@ -67,9 +71,14 @@ def main():
     stream = torch.cuda.current_stream(dev)
     sp = stream.cuda_stream
 
+    denoised = torch.empty_like(accum) if w.denoise == "bilateral" else None
+
     def step():
         ctx.render_dev(w.camera, w.width, w.height, w.spp, 0, w.depth, None, accum.data_ptr(),
                        gbuf.data_ptr(), sp)
+        if denoised is not None:
+            ctx.denoise_bilateral_dev(None, w.width, w.height, accum.data_ptr(), gbuf.data_ptr(),
+                                      denoised.data_ptr(), sp)
 
     log(f"warmup {a.warmup}")
     for _ in range(a.warmup):
@@ -105,6 +114,7 @@ def main():
             "spp": w.spp,
             "max_depth": w.depth,
             "triangles": int(len(w.buffers.triangles)),
+            "denoise": w.denoise,
         },
         "mrays_per_s": round(segments / sec / 1e6, 3),
         "segments_per_step": int(segments),
@@ -113,7 +123,10 @@ def main():
     if not a.no_roofline:
         out["roofline"] = roofline(ctx, w, step, segments)
     if not a.no_microbench:
-        out["microbench"] = microbench(ctx, w)
+        log("microbench: 16M rays on the icosphere scene")
+        mb = configs.c2(width=64, height=64, spp=1)
+        ctx.upload_scene(mb.buffers)
+        out["microbench"] = microbench(ctx, mb)
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(w, a.cpu_seconds)
     ctx.close()

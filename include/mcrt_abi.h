@@ -244,6 +244,22 @@ int mcrt_build_sphere_bvh(const GpuSphere *spheres, uint32_t n_spheres,
                           GpuBvhNode *out_nodes, uint32_t *out_n_nodes,
                           GpuSphere *out_ordered, uint32_t *out_orig_index);
 
+/* Greedy voxel meshing (src/world/mesher.rs:97-400 as driven by
+ * src/gpu.rs:386-458).  Host-only.  `blocks` holds chunks_x * chunks_z chunks
+ * of 16 x 256 x 16 u16 block ids, chunk (ix, iz) at offset
+ * (ix * chunks_z + iz) * 65536, each chunk XZY-major
+ * (x * 4096 + y * 16 + z, src/world/chunk.rs:28-57).  Chunk (ix, iz) is
+ * world chunk (cx0 + ix, cz0 + iz).  opaque[id] and block_material[id]
+ * (256 entries each) give opacity and the material slot.  With out_verts and
+ * out_tris NULL only the counts are returned; otherwise the buffers receive
+ * the mesh (MCRT_ERR_LIMIT if max_verts / max_tris are too small). */
+int mcrt_mesh_voxel_chunks(const uint16_t *blocks, uint32_t chunks_x, uint32_t chunks_z,
+                           int32_t cx0, int32_t cz0, float voxel_size,
+                           const uint8_t *opaque, const uint32_t *block_material,
+                           GpuVertex *out_verts, uint64_t max_verts,
+                           GpuTriangle *out_tris, uint64_t max_tris,
+                           uint64_t *n_verts, uint64_t *n_tris);
+
 /* Kernel timing.  When enabled, every launch of the named kernels is
  * bracketed by HIP events on its own stream; mcrt_kernel_stats returns the
  * summed milliseconds and the launch count since the last reset.

@@ -71,6 +71,11 @@ def load() -> C.CDLL:
         "mcrt_kernel_stats": (C.c_int, [_vp, C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_uint64)]),
         "mcrt_reset_stats": (C.c_int, [_vp]),
         "mcrt_render_counters": (C.c_int, [_vp, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
+        "mcrt_mesh_voxel_chunks": (
+            C.c_int,
+            [_vp, _u32, _u32, C.c_int32, C.c_int32, _f32, _vp, _vp, _vp, C.c_uint64, _vp, C.c_uint64,
+             C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)],
+        ),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -86,8 +91,35 @@ def exported_symbols() -> list:
         "mcrt_render_dev", "mcrt_denoise_bilateral", "mcrt_denoise_bilateral_dev",
         "mcrt_trace_closest", "mcrt_trace_closest_dev", "mcrt_build_mesh_bvh",
         "mcrt_build_sphere_bvh", "mcrt_set_profiling", "mcrt_kernel_stats", "mcrt_reset_stats",
-        "mcrt_render_counters",
+        "mcrt_render_counters", "mcrt_mesh_voxel_chunks",
     ]
+
+
+def mesh_voxel_chunks(blocks: np.ndarray, cx0: int, cz0: int, voxel_size: float,
+                      opaque: np.ndarray, block_material: np.ndarray):
+    """Greedy-mesh a (chunks_x, chunks_z, 16, 256, 16) uint16 block grid.
+
+    Returns (GpuVertex array, GpuTriangle array)."""
+    lib = load()
+    blocks = np.ascontiguousarray(blocks, np.uint16)
+    if blocks.ndim != 5 or blocks.shape[2:] != (16, 256, 16):
+        raise ValueError("blocks must have shape (chunks_x, chunks_z, 16, 256, 16)")
+    op = np.ascontiguousarray(opaque, np.uint8)
+    bm = np.ascontiguousarray(block_material, np.uint32)
+    if op.shape != (256,) or bm.shape != (256,):
+        raise ValueError("opaque and block_material need 256 entries")
+    nv, nt = C.c_uint64(0), C.c_uint64(0)
+    args = (_ptr(blocks), blocks.shape[0], blocks.shape[1], cx0, cz0, voxel_size, _ptr(op), _ptr(bm))
+    rc = lib.mcrt_mesh_voxel_chunks(*args, None, 0, None, 0, C.byref(nv), C.byref(nt))
+    if rc != MCRT_OK:
+        raise McrtError(rc, "mcrt_mesh_voxel_chunks rejected its input")
+    verts = np.zeros(nv.value, T.GpuVertex)
+    tris = np.zeros(nt.value, T.GpuTriangle)
+    rc = lib.mcrt_mesh_voxel_chunks(*args, _ptr(verts), len(verts), _ptr(tris), len(tris),
+                                    C.byref(nv), C.byref(nt))
+    if rc != MCRT_OK:
+        raise McrtError(rc, "mcrt_mesh_voxel_chunks failed")
+    return verts, tris
 
 
 def _ptr(a: Optional[np.ndarray]):

@@ -27,6 +27,7 @@ class Workload:
     height: int
     spp: int
     depth: int
+    denoise: Optional[str] = None
 
 
 def c1(width: int = 256, height: int = 256, spp: int = 16, depth: int = 8) -> Workload:
@@ -49,10 +50,32 @@ def c2(width: int = 1024, height: int = 1024, spp: int = 64, depth: int = 8, n: 
                     height, spp, depth)
 
 
+C3_CAMERA = "camera: {look_from: [0.0, 44.0, 46.0], look_at: [0.0, 8.0, 0.0], vfov: 55.0}\n"
+
+
+def c3(width: int = 1920, height: int = 1080, spp: int = 16, depth: int = 6, chunks: int = 32,
+       seed: int = 11) -> Workload:
+    """Voxel terrain: 512 x 256 x 512 voxels greedy-meshed to ~2M triangles,
+    gradient sky + Gaussian sun at 35 degrees elevation."""
+    from . import texture, world
+
+    text = C3_CAMERA + world.world_materials_yaml()
+    base = scene_mod.load_scene_from_str(text, "voxel-c3")
+    blocks = world.terrain_blocks(chunks, chunks, seed)
+    cx0 = -chunks // 2
+    v, t = world.mesh_terrain(blocks, cx0, cx0, base.name_to_idx)
+    del blocks
+    loaded = scene_mod.load_scene_from_str(text, "voxel-c3", extra_mesh=(v, t))
+    env = texture.rgb_to_env(texture.generate_skymap(1024, 512, sun_elevation_deg=35.0))
+    sb = scene_mod.build_scene_buffers(loaded, env=env)
+    return Workload("voxel_c3", loaded, sb, cam_mod.camera_for(loaded, width, height), width,
+                    height, spp, depth, "bilateral")
+
+
 def scene_aabb(sb: scene_mod.SceneBuffers):
     nodes = sb.mesh_bvh
     return nodes[0]["aabb_min"][:3].copy(), nodes[0]["aabb_max"][:3].copy()
 
 
 def by_name(name: str, **kw) -> Workload:
-    return {"c1": c1, "c2": c2}[name](**kw)
+    return {"c1": c1, "c2": c2, "c3": c3}[name](**kw)

@@ -57,6 +57,8 @@ struct mcrt_ctx {
     DevBuf host_accum, host_gbuf, host_out, host_accum_in;
     DevBuf trace_o, trace_d, trace_id, trace_t, trace_work;
     DevBuf spatial;
+    int spatial_radius = -1;
+    float spatial_sigma = 0.0f;
     // last render bookkeeping
     uint32_t last_batches = 0, last_depth = 0;
     uint64_t last_samples = 0;
@@ -665,14 +667,22 @@ extern "C" int mcrt_denoise_bilateral_dev(mcrt_ctx *c, const DenoiseParams *para
     if (width > 0x7FFFFFFFu || height > 0x7FFFFFFFu) return fail(c, MCRT_ERR_LIMIT, "image too large");
     hipStream_t s = (hipStream_t)stream;
     c->last_stream = s;
-    // Spatial weights exp(-dist_sq * inv2ss), denoise.wgsl:65 and :88-89.
+    // Spatial weights exp(-dist_sq * inv2ss), denoise.wgsl:65 and :88-89,
+    // uploaded once per (radius, sigma_s).  The table's device copy is
+    // rewritten only after the streams that may still read it have drained.
     const int R = p.radius, span = 2 * R + 1;
-    const float inv2ss = 1.0f / (2.0f * p.sigma_s * p.sigma_s);
-    std::vector<float> lut((size_t)span * span);
-    for (int dy = -R, k = 0; dy <= R; ++dy)
-        for (int dx = -R; dx <= R; ++dx, ++k) lut[k] = std::exp(-(float)(dx * dx + dy * dy) * inv2ss);
-    if ((rc = ensure(c, c->spatial, lut.size() * 4))) return rc;
-    HIP_TRY(c, hipMemcpyAsync(c->spatial.p, lut.data(), lut.size() * 4, hipMemcpyHostToDevice, s));
+    if (!c->spatial.p || c->spatial_radius != R || c->spatial_sigma != p.sigma_s) {
+        HIP_TRY(c, hipDeviceSynchronize());
+        const float inv2ss = 1.0f / (2.0f * p.sigma_s * p.sigma_s);
+        std::vector<float> lut((size_t)span * span);
+        for (int dy = -R, k = 0; dy <= R; ++dy)
+            for (int dx = -R; dx <= R; ++dx, ++k)
+                lut[k] = std::exp(-(float)(dx * dx + dy * dy) * inv2ss);
+        if ((rc = ensure(c, c->spatial, lut.size() * 4))) return rc;
+        HIP_TRY(c, hipMemcpy(c->spatial.p, lut.data(), lut.size() * 4, hipMemcpyHostToDevice));
+        c->spatial_radius = R;
+        c->spatial_sigma = p.sigma_s;
+    }
     BilateralArgs a{};
     a.accum = (const float4 *)accum_dev;
     a.gbuf = (const float4 *)gbuf_dev;
@@ -687,9 +697,6 @@ extern "C" int mcrt_denoise_bilateral_dev(mcrt_ctx *c, const DenoiseParams *para
         Timed t(c, "bilateral", s);
         HIP_TRY(c, launch_bilateral(a, s));
     }
-    // The LUT upload is stream-ordered before the kernel; keep the host
-    // vector alive until the copy has been consumed.
-    HIP_TRY(c, hipStreamSynchronize(s));
     return MCRT_OK;
 }
 

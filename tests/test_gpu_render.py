@@ -112,6 +112,25 @@ def test_icosphere_c2_small(ctx):
     assert same_g >= 0.999 and ok >= 0.98 and r.psnr >= 45
 
 
+@pytest.fixture(scope="module")
+def c3_small():
+    return configs.c3(width=192, height=108, spp=2)
+
+
+def test_voxel_terrain_c3_small(ctx, c3_small):
+    """Voxel terrain (2.16M triangles, sun sky) at reduced resolution."""
+    w = c3_small
+    ctx.upload_scene(w.buffers)
+    acc, gb = ctx.render(w.camera, w.width, w.height, w.spp, 0, w.depth)
+    ref, rgb = oracle.render(oracle.Scene(w.buffers), w.camera, w.width, w.height, w.spp, 0, w.depth)
+    same_g, ok, r = _parity(acc, gb, ref, rgb)
+    print(f"c3: gbuf {same_g:.4f} px {ok:.4f} psnr {r.psnr:.1f}")
+    assert same_g >= 0.999 and ok >= 0.98 and r.mse <= 0.002 and r.psnr >= 38
+    den = ctx.denoise_bilateral(acc, gb)
+    oden = oracle.denoise(acc, gb, T.default_denoise_params())
+    np.testing.assert_allclose(den[..., :3], oden[..., :3], rtol=2e-5, atol=1e-5)
+
+
 def test_segment_counter_matches_oracle(ctx):
     w = configs.c1(width=64, height=64)
     ctx.upload_scene(w.buffers)
