@@ -46,7 +46,8 @@ struct mcrt_ctx {
     hipStream_t last_stream = nullptr;
     // scene
     bool has_scene = false;
-    DevBuf mnodes, mtris, verts, tris_ref, snodes, sph, mats, env;
+    DevBuf mnodes, mtris, tshade, snodes, sph, mats, env;
+    DevBuf ovf;  // traversal-stack overflow for trees deeper than the LDS short stack
     DevScene scene{};
     uint32_t n_tris = 0, n_spheres = 0;
     // work buffers
@@ -269,9 +270,21 @@ uint32_t frames_per_batch(uint32_t wh, uint32_t spp) {
 
 int grid_for(mcrt_ctx *c, int which, uint32_t *grid) {
     int per_cu = 0;
-    hipError_t e = occupancy(which, c->scene.stack_depth, &per_cu);
+    hipError_t e = occupancy(which, c->scene.sroot != kNone, &per_cu);
     if (e != hipSuccess || per_cu < 1) per_cu = 1;
     *grid = (uint32_t)(c->num_cus * per_cu);
+    return MCRT_OK;
+}
+
+// Overflow columns for traversal stacks deeper than the LDS short stack:
+// (depth - short) entries for every thread of a persistent grid.
+int ensure_overflow(mcrt_ctx *c, uint32_t grid_threads, uint32_t **ptr) {
+    *ptr = nullptr;
+    const uint32_t depth = c->scene.stack_depth, sh = short_stack();
+    if (depth <= sh) return MCRT_OK;
+    int rc = ensure(c, c->ovf, (size_t)(depth - sh) * grid_threads * sizeof(uint32_t));
+    if (rc) return rc;
+    *ptr = (uint32_t *)c->ovf.p;
     return MCRT_OK;
 }
 
@@ -310,7 +323,7 @@ extern "C" void mcrt_destroy(mcrt_ctx *c) {
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->last_stream);
     (void)hipStreamSynchronize(c->own_stream);
-    for (DevBuf *b : {&c->mnodes, &c->mtris, &c->verts, &c->tris_ref, &c->snodes, &c->sph, &c->mats,
+    for (DevBuf *b : {&c->mnodes, &c->mtris, &c->tshade, &c->ovf, &c->snodes, &c->sph, &c->mats,
                       &c->env, &c->A[0], &c->A[1], &c->B[0], &c->B[1], &c->C[0], &c->C[1], &c->H,
                       &c->samples, &c->counters, &c->host_accum, &c->host_gbuf, &c->host_out,
                       &c->host_accum_in, &c->trace_o, &c->trace_d, &c->trace_id, &c->trace_t,
@@ -383,7 +396,15 @@ extern "C" int mcrt_upload_scene(mcrt_ctx *c, const GpuSphere *spheres, uint32_t
         dt[3 * (size_t)i + 0] = make_float4(p0[0], p0[1], p0[2], ubits(orig));
         dt[3 * (size_t)i + 1] =
             make_float4(p1[0] - p0[0], p1[1] - p0[1], p1[2] - p0[2], ubits(flags));
-        dt[3 * (size_t)i + 2] = make_float4(p2[0] - p0[0], p2[1] - p0[1], p2[2] - p0[2], ubits(i));
+        dt[3 * (size_t)i + 2] = make_float4(p2[0] - p0[0], p2[1] - p0[1], p2[2] - p0[2], 0.0f);
+    }
+    // Shading records in leaf order: vertex normals + mat_idx (with flag).
+    std::vector<float4> dsh((size_t)std::max<uint32_t>(n_tris, 1) * 3, make_float4(0, 0, 0, 0));
+    for (uint32_t i = 0; i < n_tris; ++i) {
+        for (int k = 0; k < 3; ++k) {
+            const float *nn = verts[tris[i].v[k]].normal;
+            dsh[3 * (size_t)i + k] = make_float4(nn[0], nn[1], nn[2], k == 0 ? ubits(tris[i].mat_idx) : 0.0f);
+        }
     }
     std::vector<float4> ds((size_t)std::max<uint32_t>(n_spheres, 1) * 2, make_float4(0, 0, 0, 0));
     for (uint32_t i = 0; i < n_spheres; ++i) {
@@ -398,12 +419,7 @@ extern "C" int mcrt_upload_scene(mcrt_ctx *c, const GpuSphere *spheres, uint32_t
 
     if ((rc = upload(c, c->mnodes, mt.nodes.data(), mt.nodes.size() * sizeof(float4)))) return rc;
     if ((rc = upload(c, c->mtris, dt.data(), dt.size() * sizeof(float4)))) return rc;
-    if ((rc = upload(c, c->verts, verts ? (const void *)verts : (const void *)dt.data(),
-                     verts ? (size_t)n_verts * sizeof(GpuVertex) : 0)))
-        return rc;
-    if ((rc = upload(c, c->tris_ref, tris ? (const void *)tris : (const void *)dt.data(),
-                     tris ? (size_t)n_tris * sizeof(GpuTriangle) : 0)))
-        return rc;
+    if ((rc = upload(c, c->tshade, dsh.data(), dsh.size() * sizeof(float4)))) return rc;
     if ((rc = upload(c, c->snodes, st.nodes.data(), st.nodes.size() * sizeof(float4)))) return rc;
     if ((rc = upload(c, c->sph, ds.data(), ds.size() * sizeof(float4)))) return rc;
     if ((rc = upload(c, c->mats, materials->materials, sizeof(materials->materials)))) return rc;
@@ -413,8 +429,7 @@ extern "C" int mcrt_upload_scene(mcrt_ctx *c, const GpuSphere *spheres, uint32_t
     d = DevScene{};
     d.mnodes = (const float4 *)c->mnodes.p;
     d.mtris = (const float4 *)c->mtris.p;
-    d.verts = (const GpuVertex *)c->verts.p;
-    d.tris_ref = (const uint4 *)c->tris_ref.p;
+    d.tshade = (const float4 *)c->tshade.p;
     d.mroot = n_tris ? mt.root : kNone;
     std::memcpy(d.mroot_lo, mt.lo, sizeof mt.lo);
     std::memcpy(d.mroot_hi, mt.hi, sizeof mt.hi);
@@ -474,6 +489,8 @@ extern "C" int mcrt_render_dev(mcrt_ctx *c, const CameraUniform *cam, uint32_t w
     uint32_t g_ext = 0, g_shade = 0;
     grid_for(c, 0, &g_ext);
     grid_for(c, 1, &g_shade);
+    uint32_t *ovf = nullptr;
+    if ((rc = ensure_overflow(c, g_ext * (uint32_t)trace_block(), &ovf))) return rc;
 
     const uint32_t last_frame = first_frame + spp - 1u;
     for (uint32_t bi = 0; bi < n_batches; ++bi) {
@@ -507,6 +524,8 @@ extern "C" int mcrt_render_dev(mcrt_ctx *c, const CameraUniform *cam, uint32_t w
             e.H = (float4 *)c->H.p;
             e.count = cnt + b;
             e.work = wext + b;
+            e.ovf = ovf;
+            e.ovf_stride = g_ext * (uint32_t)trace_block();
             {
                 Timed t(c, "extend", s);
                 HIP_TRY(c, launch_extend(e, g_ext, s));
@@ -623,10 +642,12 @@ extern "C" int mcrt_trace_closest_dev(mcrt_ctx *c, const float *origins_dev, con
     grid_for(c, 2, &grid);
     uint32_t need = (n + 63u) / 64u;  // waves of work
     uint32_t waves_per_block = (uint32_t)trace_block() / 64u;
-    grid = std::min(grid, (need + waves_per_block - 1) / waves_per_block);
+    grid = std::max<uint32_t>(1, std::min(grid, (need + waves_per_block - 1) / waves_per_block));
+    if ((rc = ensure_overflow(c, grid * (uint32_t)trace_block(), &a.ovf))) return rc;
+    a.ovf_stride = grid * (uint32_t)trace_block();
     {
         Timed t(c, "trace_closest", s);
-        HIP_TRY(c, launch_trace(a, std::max<uint32_t>(grid, 1), s));
+        HIP_TRY(c, launch_trace(a, grid, s));
     }
     return MCRT_OK;
 }

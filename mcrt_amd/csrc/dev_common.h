@@ -1,5 +1,6 @@
-// dev_common.h — device-side scene layout and arithmetic shared by the
-// render, trace and adjoint kernels of libmcrt_hip.so (gfx950, wave64).
+// dev_common.h — device-side scene layout, arithmetic and the persistent
+// closest-hit traversal shared by the render, trace and adjoint kernels of
+// libmcrt_hip.so (gfx950, wave64).
 //
 // The arithmetic restates src/shaders/path_trace.wgsl operation by operation
 // (the file is compiled with -ffp-contract=off, so nothing is fused), which
@@ -34,20 +35,23 @@ constexpr uint32_t kNone = 0xFFFFFFFFu;
 constexpr uint32_t kPrimLast = 1u;
 constexpr uint32_t kPrimCull = 2u;
 
-// Triangle (48 B, leaf order):
+// Triangle (48 B, leaf order), used by traversal:
 //   t0 = (v0.xyz, orig_id bits)  t1 = (e1.xyz, flags bits)
-//   t2 = (e2.xyz, ordered index bits)
+//   t2 = (e2.xyz, -)
 // e1 = v1 - v0 and e2 = v2 - v0 are formed on the host in FP32, the same
 // rounding the shader applies at :302-303.
+// Triangle shading record (48 B, leaf order), used for the back-face test
+// and by the shade pass:
+//   n0 = (normal of v0, mat_idx bits incl. the cull flag)
+//   n1 = (normal of v1, -)  n2 = (normal of v2, -)
 // Sphere (32 B, leaf order):
 //   s0 = (centre.xyz, radius)  s1 = (mat, orig_id, flags, -) as uint bits
 
 struct DevScene {
     const float4 *mnodes;
     const float4 *mtris;
-    const GpuVertex *verts;
-    const uint4 *tris_ref;  // ordered GpuTriangle
-    uint32_t mroot;         // kNone = no mesh
+    const float4 *tshade;
+    uint32_t mroot;  // kNone = no mesh
     float mroot_lo[3], mroot_hi[3];
     const float4 *snodes;
     const float4 *sph;
@@ -56,7 +60,7 @@ struct DevScene {
     const GpuMaterial *mats;
     const float4 *env;
     uint32_t env_w, env_h;
-    uint32_t stack_depth;  // LDS stack entries per lane
+    uint32_t stack_depth;  // traversal stack entries needed per lane
 };
 
 // ---------------------------------------------------------------------------
@@ -107,8 +111,8 @@ constexpr float kTau = 6.283185307179586f;
 constexpr float kPi = 3.141592653589793f;
 
 // ---------------------------------------------------------------------------
-// Slab test on a child box, path_trace.wgsl:248-256.  inv = 1/dir computed
-// once per ray (same value the shader recomputes per call).
+// Slab test on a box, path_trace.wgsl:248-256.  inv = 1/dir computed once
+// per ray (the value the shader recomputes per call).
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ void slab2(V3 o, V3 inv, float lox, float hix, float loy,
                                       float hiy, float loz, float hiz, float &te,
@@ -160,36 +164,47 @@ __device__ __forceinline__ float sphere_t(V3 o, V3 d, float4 s0, float t_min, fl
     return t;
 }
 
-// Interpolated shading normal of an ordered triangle at (u, v),
-// path_trace.wgsl:448-450.
+// Interpolated shading normal of a leaf-order triangle at (u, v),
+// path_trace.wgsl:448-450.  mat receives mat_idx with its cull flag.
 __device__ __forceinline__ V3 tri_normal(const DevScene &sc, uint32_t ordered, float u, float v,
-                                         uint4 &tri) {
-    tri = sc.tris_ref[ordered];
-    const GpuVertex &va = sc.verts[tri.x];
-    const GpuVertex &vb = sc.verts[tri.y];
-    const GpuVertex &vc = sc.verts[tri.z];
+                                         uint32_t &mat) {
+    const float4 *r = sc.tshade + 3u * ordered;
+    float4 a = r[0], b = r[1], c = r[2];
+    mat = __float_as_uint(a.w);
     float w = 1.0f - u - v;
-    return normalize((w * ld3(va.normal) + u * ld3(vb.normal)) + v * ld3(vc.normal));
+    return normalize((w * xyz(a) + u * xyz(b)) + v * xyz(c));
 }
 
 // ---------------------------------------------------------------------------
-// Closest hit over one child-pair BVH.
+// Persistent closest-hit traversal.
 //
 // Visit rule (identical to the oracle's order-free closest hit): a child is
 // entered when the reference slab test passes against the query interval
 // [t_min, t_max) and its entry distance does not exceed best_t*(1+2^-10).
 // Accept rule: lowest (t, original id) lexicographically.  Both rules are
-// independent of visit order, so near-first ordering and the LDS stack leave
-// the result unchanged.
+// independent of visit order, so near-first ordering, the stack layout and
+// the scheduling of rays onto lanes leave every result unchanged.
 //
-// The stack lives in LDS, one column per lane: stack[i * kBlock + tid].
+// Scheduling: a wave pulls chunks of kChunk ray slots with one atomic and
+// hands them out to idle lanes whenever at least kRefill lanes are idle, so
+// lanes whose rays finished early are refilled instead of idling until the
+// slowest ray of the wave ends.  Inside an iteration every lane first walks
+// internal nodes until it reaches a leaf (or finishes), then the leaves are
+// intersected together (the "while-while" order).
+//
+// Stack: the first kShortStack entries of each lane live in LDS
+// (lds[i * kBlock + tid]); deeper entries go to a per-lane column of a
+// global overflow buffer (ovf[(i - kShortStack) * ovf_stride + gtid]).
 // ---------------------------------------------------------------------------
 constexpr float kCullSlack = 1.0009765625f;  // 1 + 2^-10
+constexpr uint32_t kShortStack = 16;
+constexpr uint32_t kChunk = 256;
+constexpr uint32_t kRefill = 16;
 
 struct Best {
     float t;
     uint32_t id;    // original id (for the tie rule / reporting)
-    uint32_t prim;  // leaf-order index (shading)
+    uint32_t prim;  // leaf-order index; kLeafBit marks a sphere
     float u, v;
 };
 
@@ -197,132 +212,211 @@ __device__ __forceinline__ bool better(float t, uint32_t id, const Best &b) {
     return t < b.t || (t == b.t && id < b.id);
 }
 
-template <int kBlock, bool kSphere>
-__device__ __forceinline__ void traverse(const DevScene &sc, V3 o, V3 d, V3 inv, float t_min,
-                                         float t_max, Best &best, uint32_t *stack,
-                                         uint32_t &work_nodes) {
-    const uint32_t root = kSphere ? sc.sroot : sc.mroot;
-    if (root == kNone) return;
-    {
-        const float *lo = kSphere ? sc.sroot_lo : sc.mroot_lo;
-        const float *hi = kSphere ? sc.sroot_hi : sc.mroot_hi;
-        float te, tx;
-        slab2(o, inv, lo[0], hi[0], lo[1], hi[1], lo[2], hi[2], te, tx);
-        if (!(tx > fmaxf(te, t_min) && te < t_max)) return;
+struct Stack {
+    uint32_t *lds;
+    uint32_t *ovf;
+    uint32_t ovf_stride, tid, gtid, block;
+    __device__ __forceinline__ void set(uint32_t i, uint32_t v) const {
+        if (i < kShortStack)
+            lds[i * block + tid] = v;
+        else
+            ovf[(size_t)(i - kShortStack) * ovf_stride + gtid] = v;
     }
-    const float4 *nodes = kSphere ? sc.snodes : sc.mnodes;
-    const uint32_t tid = threadIdx.x;
-    uint32_t sp = 0;
-    uint32_t cur = root;
-    for (;;) {
-        if (!(cur & kLeafBit)) {
-            ++work_nodes;
-            const float4 *n = nodes + 4u * cur;
-            float4 q0 = n[0], q1 = n[1], q2 = n[2], q3 = n[3];
-            float te0, tx0, te1, tx1;
-            slab2(o, inv, q0.x, q0.y, q0.z, q0.w, q2.x, q2.y, te0, tx0);
-            slab2(o, inv, q1.x, q1.y, q1.z, q1.w, q2.z, q2.w, te1, tx1);
-            const float lim = best.t * kCullSlack;
-            bool h0 = tx0 > fmaxf(te0, t_min) && te0 < t_max && !(te0 > lim);
-            bool h1 = tx1 > fmaxf(te1, t_min) && te1 < t_max && !(te1 > lim);
-            uint32_t c0 = __float_as_uint(q3.x), c1 = __float_as_uint(q3.y);
-            if (h0 && h1) {
-                uint32_t nearc = c0, farc = c1;
-                if (te1 < te0) {
-                    nearc = c1;
-                    farc = c0;
-                }
-                stack[sp * kBlock + tid] = farc;
-                ++sp;
-                cur = nearc;
-                continue;
-            }
-            if (h0) {
-                cur = c0;
-                continue;
-            }
-            if (h1) {
-                cur = c1;
-                continue;
-            }
-        } else {
-            uint32_t i = cur & ~kLeafBit;
-            if (!kSphere) {
-                for (;;) {
-                    const float4 *tp = sc.mtris + 3u * i;
-                    float4 a = tp[0], b = tp[1], c = tp[2];
-                    uint32_t flags = __float_as_uint(b.w);
-                    float t, u, v;
-                    if (mt_hit(o, d, xyz(a), xyz(b), xyz(c), t_min, t_max, t, u, v)) {
-                        uint32_t id = __float_as_uint(a.w);
-                        if (best.prim == kNone || better(t, id, best)) {
-                            bool ok = true;
-                            uint32_t ordered = __float_as_uint(c.w);
-                            if (flags & kPrimCull) {
-                                uint4 tri;
-                                V3 nn = tri_normal(sc, ordered, u, v, tri);
-                                ok = dot(d, nn) < 0.0f;
-                            }
-                            if (ok) {
-                                best.t = t;
-                                best.id = id;
-                                best.prim = ordered;
-                                best.u = u;
-                                best.v = v;
-                            }
-                        }
-                    }
-                    if (flags & kPrimLast) break;
-                    ++i;
-                }
-            } else {
-                for (;;) {
-                    const float4 *spp = sc.sph + 2u * i;
-                    float4 s0 = spp[0], s1 = spp[1];
-                    uint32_t flags = __float_as_uint(s1.z);
-                    float t = sphere_t(o, d, s0, t_min, t_max);
-                    if (t > 0.0f) {
-                        uint32_t id = kLeafBit | __float_as_uint(s1.y);
-                        if (best.prim == kNone || better(t, id, best)) {
-                            best.t = t;
-                            best.id = id;
-                            best.prim = kLeafBit | i;
-                            best.u = 0.0f;
-                            best.v = 0.0f;
-                        }
-                    }
-                    if (flags & kPrimLast) break;
-                    ++i;
-                }
-            }
+    __device__ __forceinline__ uint32_t get(uint32_t i) const {
+        return i < kShortStack ? lds[i * block + tid] : ovf[(size_t)(i - kShortStack) * ovf_stride + gtid];
+    }
+};
+
+// Root entry of one BVH for a ray: the root box test of the reference
+// (the root node is popped and slab-tested first, :375-379).
+__device__ __forceinline__ uint32_t enter_root(const DevScene &sc, bool sphere, V3 o, V3 inv,
+                                               float t_min, float t_max) {
+    const uint32_t root = sphere ? sc.sroot : sc.mroot;
+    if (root == kNone) return kNone;
+    const float *lo = sphere ? sc.sroot_lo : sc.mroot_lo;
+    const float *hi = sphere ? sc.sroot_hi : sc.mroot_hi;
+    float te, tx;
+    slab2(o, inv, lo[0], hi[0], lo[1], hi[1], lo[2], hi[2], te, tx);
+    return (tx > fmaxf(te, t_min) && te < t_max) ? root : kNone;
+}
+
+// One internal-node step: slab-test both children, descend near-first.
+__device__ __forceinline__ void node_step(const float4 *nodes, V3 o, V3 inv, float t_min,
+                                          float t_max, float best_t, uint32_t &cur, uint32_t &sp,
+                                          const Stack &st) {
+    const float4 *n = nodes + 4u * cur;
+    float4 q0 = n[0], q1 = n[1], q2 = n[2], q3 = n[3];
+    float te0, tx0, te1, tx1;
+    slab2(o, inv, q0.x, q0.y, q0.z, q0.w, q2.x, q2.y, te0, tx0);
+    slab2(o, inv, q1.x, q1.y, q1.z, q1.w, q2.z, q2.w, te1, tx1);
+    const float lim = best_t * kCullSlack;
+    bool h0 = tx0 > fmaxf(te0, t_min) && te0 < t_max && !(te0 > lim);
+    bool h1 = tx1 > fmaxf(te1, t_min) && te1 < t_max && !(te1 > lim);
+    uint32_t c0 = __float_as_uint(q3.x), c1 = __float_as_uint(q3.y);
+    if (h0 && h1) {
+        uint32_t nearc = c0, farc = c1;
+        if (te1 < te0) {
+            nearc = c1;
+            farc = c0;
         }
-        if (sp == 0) break;
-        --sp;
-        cur = stack[sp * kBlock + tid];
+        st.set(sp, farc);
+        ++sp;
+        cur = nearc;
+    } else if (h0) {
+        cur = c0;
+    } else if (h1) {
+        cur = c1;
+    } else {
+        cur = sp ? st.get(--sp) : kNone;
     }
 }
 
-// world_hit, path_trace.wgsl:471-479: spheres first, then the mesh against
-// [t_min, sphere t) — the sphere wins a tie.
-template <int kBlock>
-__device__ __forceinline__ Best world_hit(const DevScene &sc, V3 o, V3 d, float t_min,
-                                          float t_max, uint32_t *stack, uint32_t &work) {
-    V3 inv = v3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
-    Best sb;
-    sb.t = t_max;
-    sb.id = kNone;
-    sb.prim = kNone;
-    sb.u = sb.v = 0.0f;
-    traverse<kBlock, true>(sc, o, d, inv, t_min, t_max, sb, stack, work);
-    float mesh_tmax = sb.prim != kNone ? sb.t : t_max;
-    Best mb;
-    mb.t = mesh_tmax;
-    mb.id = kNone;
-    mb.prim = kNone;
-    mb.u = mb.v = 0.0f;
-    traverse<kBlock, false>(sc, o, d, inv, t_min, mesh_tmax, mb, stack, work);
-    if (mb.prim != kNone) return mb;
-    return sb;
+__device__ __forceinline__ void tri_leaf(const DevScene &sc, uint32_t i, V3 o, V3 d, float t_min,
+                                         float t_max, Best &best) {
+    for (;;) {
+        const float4 *tp = sc.mtris + 3u * i;
+        float4 a = tp[0], b = tp[1], c = tp[2];
+        uint32_t flags = __float_as_uint(b.w);
+        float t, u, v;
+        if (mt_hit(o, d, xyz(a), xyz(b), xyz(c), t_min, t_max, t, u, v)) {
+            uint32_t id = __float_as_uint(a.w);
+            if (best.prim == kNone || better(t, id, best)) {
+                bool ok = true;
+                if (flags & kPrimCull) {  // :438-452, culled candidates do not shrink t
+                    uint32_t mat;
+                    V3 nn = tri_normal(sc, i, u, v, mat);
+                    ok = dot(d, nn) < 0.0f;
+                }
+                if (ok) {
+                    best.t = t;
+                    best.id = id;
+                    best.prim = i;
+                    best.u = u;
+                    best.v = v;
+                }
+            }
+        }
+        if (flags & kPrimLast) break;
+        ++i;
+    }
+}
+
+__device__ __forceinline__ void sphere_leaf(const DevScene &sc, uint32_t i, V3 o, V3 d,
+                                            float t_min, float t_max, Best &best) {
+    for (;;) {
+        const float4 *p = sc.sph + 2u * i;
+        float4 s0 = p[0], s1 = p[1];
+        uint32_t flags = __float_as_uint(s1.z);
+        float t = sphere_t(o, d, s0, t_min, t_max);
+        if (t > 0.0f) {
+            uint32_t id = kLeafBit | __float_as_uint(s1.y);
+            if (best.prim == kNone || better(t, id, best)) {
+                best.t = t;
+                best.id = id;
+                best.prim = kLeafBit | i;
+                best.u = 0.0f;
+                best.v = 0.0f;
+            }
+        }
+        if (flags & kPrimLast) break;
+        ++i;
+    }
+}
+
+// IO policy: count(), work(), load(slot, o, d, t_min, t_max), store(slot, best).
+// kSpheres = false compiles the variant for scenes without spheres (no
+// sphere phase state), launched when the sphere BVH is empty.
+template <int kBlock, bool kSpheres, class IO>
+__device__ __forceinline__ void trace_persistent(const DevScene &sc, const IO &io, uint32_t *lds,
+                                                 uint32_t *ovf, uint32_t ovf_stride) {
+    const uint32_t lane = __lane_id();
+    Stack st{lds, ovf, ovf_stride, threadIdx.x, blockIdx.x * kBlock + threadIdx.x, (uint32_t)kBlock};
+    const uint32_t total = io.count();
+    uint32_t wbase = 0, wend = 0;  // wave-local slot range
+    bool more = true;
+    // lane state
+    uint32_t slot = kNone;
+    V3 o = v3(0, 0, 0), d = v3(0, 0, 0), inv = v3(0, 0, 0);
+    float t_min = 0.0f, t_q = 0.0f, t_cur = 0.0f;  // query and current-phase upper bounds
+    uint32_t cur = kNone, sp = 0;
+    bool sphere_phase = false;
+    Best best, sbest;
+    best.t = 0.0f;
+    best.id = best.prim = kNone;
+    best.u = best.v = 0.0f;
+    sbest = best;
+    for (;;) {
+        // ---- refill idle lanes --------------------------------------------
+        const uint64_t idle = __ballot(slot == kNone);
+        const uint32_t nidle = (uint32_t)__popcll(idle);
+        if (nidle >= kRefill) {
+            if (wbase >= wend && more) {
+                uint32_t g = 0;
+                if (lane == 0) g = atomicAdd(io.work(), kChunk);
+                g = __shfl(g, 0);
+                if (g < total) {
+                    wbase = g;
+                    wend = min(g + kChunk, total);
+                } else {
+                    more = false;
+                }
+            }
+            const uint32_t avail = wend - wbase;
+            const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(idle >> 32),
+                                                            __builtin_amdgcn_mbcnt_lo((uint32_t)idle, 0u));
+            if (slot == kNone && rank < avail) {
+                slot = wbase + rank;
+                io.load(slot, o, d, t_min, t_q);
+                inv = v3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+                best.prim = kNone;
+                best.id = kNone;
+                best.t = t_q;
+                if (kSpheres) sbest = best;
+                sphere_phase = kSpheres && sc.sroot != kNone;
+                t_cur = t_q;
+                cur = enter_root(sc, sphere_phase, o, inv, t_min, t_cur);
+                sp = 0;
+            }
+            wbase += min(avail, nidle);
+        }
+        if (__ballot(slot != kNone) == 0) {
+            if (!more && wbase >= wend) break;
+            continue;
+        }
+        // ---- internal nodes until every lane holds a leaf or is done -------
+        for (;;) {
+            const bool in = slot != kNone && cur != kNone && !(cur & kLeafBit);
+            if (__ballot(in) == 0) break;
+            if (in) {
+                const float4 *nodes = (kSpheres && sphere_phase) ? sc.snodes : sc.mnodes;
+                node_step(nodes, o, inv, t_min, t_cur, best.t, cur, sp, st);
+            }
+        }
+        // ---- leaves ----------------------------------------------------------
+        if (slot != kNone && cur != kNone) {
+            if (kSpheres && sphere_phase)
+                sphere_leaf(sc, cur & ~kLeafBit, o, d, t_min, t_cur, best);
+            else
+                tri_leaf(sc, cur & ~kLeafBit, o, d, t_min, t_cur, best);
+            cur = sp ? st.get(--sp) : kNone;
+        }
+        // ---- phase end: spheres -> mesh (world_hit, :471-479) or finish ----
+        if (slot != kNone && cur == kNone) {
+            if (kSpheres && sphere_phase) {
+                sbest = best;
+                sphere_phase = false;
+                t_cur = sbest.prim != kNone ? sbest.t : t_q;
+                best.prim = kNone;
+                best.id = kNone;
+                best.t = t_cur;
+                cur = enter_root(sc, false, o, inv, t_min, t_cur);
+            }
+            if (!sphere_phase && cur == kNone) {
+                io.store(slot, (!kSpheres || best.prim != kNone) ? best : sbest);
+                slot = kNone;
+            }
+        }
+    }
 }
 
 }  // namespace mcrt

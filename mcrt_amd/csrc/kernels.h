@@ -23,6 +23,8 @@ struct ExtArgs {
     float4 *H;
     const uint32_t *count;
     uint32_t *work;
+    uint32_t *ovf;  // traversal-stack overflow, (depth - short) x ovf_stride
+    uint32_t ovf_stride;  // threads in the persistent grid
 };
 
 struct ShadeArgs {
@@ -54,6 +56,8 @@ struct TraceArgs {
     uint32_t *out_id;
     float *out_t;
     uint32_t *work;
+    uint32_t *ovf;
+    uint32_t ovf_stride;
 };
 
 struct BilateralArgs {
@@ -73,7 +77,8 @@ hipError_t launch_accumulate(const AccArgs &a, hipStream_t s);
 hipError_t launch_trace(const TraceArgs &a, uint32_t grid, hipStream_t s);
 hipError_t launch_bilateral(const BilateralArgs &a, hipStream_t s);
 // which: 0 extend, 1 shade, 2 trace
-hipError_t occupancy(int which, uint32_t stack_depth, int *blocks_per_cu);
+hipError_t occupancy(int which, bool spheres, int *blocks_per_cu);
 int trace_block();
+uint32_t short_stack();
 
 }  // namespace mcrt

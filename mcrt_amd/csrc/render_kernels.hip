@@ -106,27 +106,27 @@ __device__ __forceinline__ uint32_t grab64(uint32_t *work) {
 // ---------------------------------------------------------------------------
 // k_extend: closest hit of every queued path (world_hit at [T_MIN, T_MAX))
 // ---------------------------------------------------------------------------
-template <int kBlock>
-__global__ __launch_bounds__(kBlock) void k_extend(ExtArgs a) {
-    extern __shared__ uint32_t lds_stack[];
-    const uint32_t count = *a.count;
-    uint32_t work = 0;
-    for (;;) {
-        uint32_t base = grab64(a.work);
-        if (base >= count) break;
-        uint32_t slot = base + __lane_id();
-        if (slot < count) {
-            float4 A = a.A[slot], B = a.B[slot];
-            Best b = world_hit<kBlock>(a.scene, xyz(A), xyz(B), MCRT_T_MIN, MCRT_T_MAX,
-                                       lds_stack, work);
-            float4 h;
-            if (b.prim != kNone)
-                h = make_float4(b.t, b.u, b.v, __uint_as_float(b.prim));
-            else
-                h = make_float4(-1.0f, 0.0f, 0.0f, __uint_as_float(kNone));
-            a.H[slot] = h;
-        }
+struct ExtIO {
+    const ExtArgs &a;
+    __device__ uint32_t count() const { return *a.count; }
+    __device__ uint32_t *work() const { return a.work; }
+    __device__ void load(uint32_t slot, V3 &o, V3 &d, float &tmin, float &tmax) const {
+        float4 A = a.A[slot], B = a.B[slot];
+        o = xyz(A);
+        d = xyz(B);
+        tmin = MCRT_T_MIN;
+        tmax = MCRT_T_MAX;
     }
+    __device__ void store(uint32_t slot, const Best &b) const {
+        a.H[slot] = b.prim != kNone ? make_float4(b.t, b.u, b.v, __uint_as_float(b.prim))
+                                    : make_float4(-1.0f, 0.0f, 0.0f, __uint_as_float(kNone));
+    }
+};
+
+template <int kBlock, bool kSpheres>
+__global__ __launch_bounds__(kBlock) void k_extend(ExtArgs a) {
+    __shared__ uint32_t lds[kShortStack * kBlock];
+    trace_persistent<kBlock, kSpheres>(a.scene, ExtIO{a}, lds, a.ovf, a.ovf_stride);
 }
 
 // ---------------------------------------------------------------------------
@@ -153,11 +153,11 @@ __device__ __forceinline__ Surface surface(const DevScene &sc, V3 o, V3 d, float
         s.n = s.front ? outn : -outn;
         s.mat = __float_as_uint(s1.x);
     } else {  // triangle, :445-458
-        uint4 tri;
-        V3 n = tri_normal(sc, prim, h.y, h.z, tri);
+        uint32_t mat;
+        V3 n = tri_normal(sc, prim, h.y, h.z, mat);
         s.front = dot(d, n) < 0.0f;
         s.n = s.front ? n : -n;
-        s.mat = tri.w & 0x7FFFFFFFu;
+        s.mat = mat & 0x7FFFFFFFu;
     }
     return s;
 }
@@ -281,22 +281,26 @@ __global__ __launch_bounds__(256) void k_accumulate(AccArgs a) {
 // ---------------------------------------------------------------------------
 // k_trace: standalone closest-hit query (mcrt_trace_closest)
 // ---------------------------------------------------------------------------
-template <int kBlock>
-__global__ __launch_bounds__(kBlock) void k_trace(TraceArgs a) {
-    extern __shared__ uint32_t lds_stack[];
-    uint32_t work = 0;
-    for (;;) {
-        uint32_t base = grab64(a.work);
-        if (base >= a.n) break;
-        uint32_t i = base + __lane_id();
-        if (i < a.n) {
-            V3 o = v3(a.origins[3u * i], a.origins[3u * i + 1u], a.origins[3u * i + 2u]);
-            V3 d = v3(a.dirs[3u * i], a.dirs[3u * i + 1u], a.dirs[3u * i + 2u]);
-            Best b = world_hit<kBlock>(a.scene, o, d, a.tmin, a.tmax, lds_stack, work);
-            a.out_id[i] = b.prim != kNone ? b.id : MCRT_ID_MISS;
-            a.out_t[i] = b.prim != kNone ? b.t : -1.0f;
-        }
+struct TraceIO {
+    const TraceArgs &a;
+    __device__ uint32_t count() const { return a.n; }
+    __device__ uint32_t *work() const { return a.work; }
+    __device__ void load(uint32_t i, V3 &o, V3 &d, float &tmin, float &tmax) const {
+        o = v3(a.origins[3u * i], a.origins[3u * i + 1u], a.origins[3u * i + 2u]);
+        d = v3(a.dirs[3u * i], a.dirs[3u * i + 1u], a.dirs[3u * i + 2u]);
+        tmin = a.tmin;
+        tmax = a.tmax;
     }
+    __device__ void store(uint32_t i, const Best &b) const {
+        a.out_id[i] = b.prim != kNone ? b.id : MCRT_ID_MISS;
+        a.out_t[i] = b.prim != kNone ? b.t : -1.0f;
+    }
+};
+
+template <int kBlock, bool kSpheres>
+__global__ __launch_bounds__(kBlock) void k_trace(TraceArgs a) {
+    __shared__ uint32_t lds[kShortStack * kBlock];
+    trace_persistent<kBlock, kSpheres>(a.scene, TraceIO{a}, lds, a.ovf, a.ovf_stride);
 }
 
 __global__ void k_fill_u32(uint32_t *p, uint32_t n, uint32_t v) {
@@ -320,8 +324,10 @@ hipError_t launch_generate(const GenArgs &a, hipStream_t s) {
 }
 
 hipError_t launch_extend(const ExtArgs &a, uint32_t grid, hipStream_t s) {
-    size_t lds = (size_t)a.scene.stack_depth * kTraceBlock * sizeof(uint32_t);
-    hipLaunchKernelGGL(k_extend<kTraceBlock>, dim3(grid), dim3(kTraceBlock), lds, s, a);
+    if (a.scene.sroot != kNone)
+        hipLaunchKernelGGL((k_extend<kTraceBlock, true>), dim3(grid), dim3(kTraceBlock), 0, s, a);
+    else
+        hipLaunchKernelGGL((k_extend<kTraceBlock, false>), dim3(grid), dim3(kTraceBlock), 0, s, a);
     return hipGetLastError();
 }
 
@@ -336,25 +342,27 @@ hipError_t launch_accumulate(const AccArgs &a, hipStream_t s) {
 }
 
 hipError_t launch_trace(const TraceArgs &a, uint32_t grid, hipStream_t s) {
-    size_t lds = (size_t)a.scene.stack_depth * kTraceBlock * sizeof(uint32_t);
-    hipLaunchKernelGGL(k_trace<kTraceBlock>, dim3(grid), dim3(kTraceBlock), lds, s, a);
+    if (a.scene.sroot != kNone)
+        hipLaunchKernelGGL((k_trace<kTraceBlock, true>), dim3(grid), dim3(kTraceBlock), 0, s, a);
+    else
+        hipLaunchKernelGGL((k_trace<kTraceBlock, false>), dim3(grid), dim3(kTraceBlock), 0, s, a);
     return hipGetLastError();
 }
 
-hipError_t occupancy(int which, uint32_t stack_depth, int *blocks_per_cu) {
-    size_t lds = (size_t)stack_depth * kTraceBlock * sizeof(uint32_t);
-    switch (which) {
-    case 0:
-        return hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            blocks_per_cu, reinterpret_cast<const void *>(&k_extend<kTraceBlock>), kTraceBlock, lds);
-    case 1:
-        return hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            blocks_per_cu, reinterpret_cast<const void *>(&k_shade), 256, 0);
-    default:
-        return hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            blocks_per_cu, reinterpret_cast<const void *>(&k_trace<kTraceBlock>), kTraceBlock, lds);
-    }
+hipError_t occupancy(int which, bool spheres, int *blocks_per_cu) {
+    const void *f = nullptr;
+    if (which == 0)
+        f = spheres ? reinterpret_cast<const void *>(&k_extend<kTraceBlock, true>)
+                    : reinterpret_cast<const void *>(&k_extend<kTraceBlock, false>);
+    else if (which == 1)
+        f = reinterpret_cast<const void *>(&k_shade);
+    else
+        f = spheres ? reinterpret_cast<const void *>(&k_trace<kTraceBlock, true>)
+                    : reinterpret_cast<const void *>(&k_trace<kTraceBlock, false>);
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, f, which == 1 ? 256 : kTraceBlock, 0);
 }
+
+uint32_t short_stack() { return kShortStack; }
 
 int trace_block() { return kTraceBlock; }
 
