@@ -244,6 +244,38 @@ int mcrt_build_sphere_bvh(const GpuSphere *spheres, uint32_t n_spheres,
                           GpuBvhNode *out_nodes, uint32_t *out_n_nodes,
                           GpuSphere *out_ordered, uint32_t *out_orig_index);
 
+/* Differentiable render + path-replay adjoint.  No counterpart in the
+ * reference: the reference estimator (path_trace.wgsl:583-662) extended with
+ * up to 8 "parameter materials" (Lambertian, RGB albedo, white emission added
+ * at every vertex on them; with emission 0 exactly the reference Lambertian).
+ *  - params: n_params x (albedo.r, albedo.g, albedo.b, emission), host memory.
+ *  - param_of_material: 64 entries, material slot -> parameter index or -1
+ *    (the material table's own entry is used).
+ *  - image: W*H RGBA32F, the mean of frames first_frame..first_frame+spp-1.
+ *  - d_image: W*H RGBA32F, dLoss/dimage (alpha ignored).
+ *  - grad: n_params x 4 floats, dLoss/dparams, summed in a fixed order.
+ * The adjoint re-traces every sample twice with its PCG stream (no tape);
+ * replayed vertices need non-zero attenuation. */
+int mcrt_diff_render(mcrt_ctx *ctx, const CameraUniform *camera, uint32_t width,
+                     uint32_t height, uint32_t spp, uint32_t first_frame,
+                     uint32_t max_depth, const float *params, uint32_t n_params,
+                     const int32_t *param_of_material, float *out_image);
+int mcrt_diff_render_dev(mcrt_ctx *ctx, const CameraUniform *camera, uint32_t width,
+                         uint32_t height, uint32_t spp, uint32_t first_frame,
+                         uint32_t max_depth, const float *params, uint32_t n_params,
+                         const int32_t *param_of_material, float *out_image_dev,
+                         void *stream);
+int mcrt_diff_adjoint(mcrt_ctx *ctx, const CameraUniform *camera, uint32_t width,
+                      uint32_t height, uint32_t spp, uint32_t first_frame,
+                      uint32_t max_depth, const float *params, uint32_t n_params,
+                      const int32_t *param_of_material, const float *d_image,
+                      float *out_grad);
+int mcrt_diff_adjoint_dev(mcrt_ctx *ctx, const CameraUniform *camera, uint32_t width,
+                          uint32_t height, uint32_t spp, uint32_t first_frame,
+                          uint32_t max_depth, const float *params, uint32_t n_params,
+                          const int32_t *param_of_material, const float *d_image_dev,
+                          float *out_grad_dev, void *stream);
+
 /* Greedy voxel meshing (src/world/mesher.rs:97-400 as driven by
  * src/gpu.rs:386-458).  Host-only.  `blocks` holds chunks_x * chunks_z chunks
  * of 16 x 256 x 16 u16 block ids, chunk (ix, iz) at offset

@@ -71,6 +71,11 @@ def load() -> C.CDLL:
         "mcrt_kernel_stats": (C.c_int, [_vp, C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_uint64)]),
         "mcrt_reset_stats": (C.c_int, [_vp]),
         "mcrt_render_counters": (C.c_int, [_vp, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
+        "mcrt_diff_render": (C.c_int, [_vp, _vp, _u32, _u32, _u32, _u32, _u32, _vp, _u32, _vp, _vp]),
+        "mcrt_diff_render_dev": (C.c_int, [_vp, _vp, _u32, _u32, _u32, _u32, _u32, _vp, _u32, _vp, _vp, _vp]),
+        "mcrt_diff_adjoint": (C.c_int, [_vp, _vp, _u32, _u32, _u32, _u32, _u32, _vp, _u32, _vp, _vp, _vp]),
+        "mcrt_diff_adjoint_dev": (C.c_int, [_vp, _vp, _u32, _u32, _u32, _u32, _u32, _vp, _u32, _vp, _vp, _vp,
+                                            _vp]),
         "mcrt_mesh_voxel_chunks": (
             C.c_int,
             [_vp, _u32, _u32, C.c_int32, C.c_int32, _f32, _vp, _vp, _vp, C.c_uint64, _vp, C.c_uint64,
@@ -91,7 +96,8 @@ def exported_symbols() -> list:
         "mcrt_render_dev", "mcrt_denoise_bilateral", "mcrt_denoise_bilateral_dev",
         "mcrt_trace_closest", "mcrt_trace_closest_dev", "mcrt_build_mesh_bvh",
         "mcrt_build_sphere_bvh", "mcrt_set_profiling", "mcrt_kernel_stats", "mcrt_reset_stats",
-        "mcrt_render_counters", "mcrt_mesh_voxel_chunks",
+        "mcrt_render_counters", "mcrt_mesh_voxel_chunks", "mcrt_diff_render", "mcrt_diff_render_dev",
+        "mcrt_diff_adjoint", "mcrt_diff_adjoint_dev",
     ]
 
 
@@ -261,6 +267,45 @@ class Context:
     def trace_closest_dev(self, o_ptr, d_ptr, n, tmin, tmax, id_ptr, t_ptr, stream_ptr) -> None:
         self._check(self._lib.mcrt_trace_closest_dev(self._h, o_ptr, d_ptr, n, tmin, tmax, id_ptr,
                                                      t_ptr, stream_ptr))
+
+    # --- differentiable render (extended estimator, see include/mcrt_abi.h) ---
+    @staticmethod
+    def _diff_inputs(camera, params, pmap):
+        cam = np.ascontiguousarray(camera, T.CameraUniform)
+        p = np.ascontiguousarray(params, np.float32).reshape(-1, 4)
+        m = np.ascontiguousarray(pmap, np.int32)
+        if m.shape != (64,):
+            raise ValueError("param_of_material needs 64 entries")
+        return cam, p, m
+
+    def diff_render(self, camera, width, height, spp, first_frame, max_depth, params, pmap):
+        cam, p, m = self._diff_inputs(camera, params, pmap)
+        out = np.empty((height, width, 4), np.float32)
+        self._check(self._lib.mcrt_diff_render(self._h, _ptr(cam), width, height, spp, first_frame,
+                                               max_depth, _ptr(p), len(p), _ptr(m), _ptr(out)))
+        return out
+
+    def diff_adjoint(self, camera, width, height, spp, first_frame, max_depth, params, pmap, d_image):
+        cam, p, m = self._diff_inputs(camera, params, pmap)
+        di = np.ascontiguousarray(d_image, np.float32)
+        g = np.empty((len(p), 4), np.float32)
+        self._check(self._lib.mcrt_diff_adjoint(self._h, _ptr(cam), width, height, spp, first_frame,
+                                                max_depth, _ptr(p), len(p), _ptr(m), _ptr(di), _ptr(g)))
+        return g
+
+    def diff_render_dev(self, camera, width, height, spp, first_frame, max_depth, params, pmap,
+                        out_ptr, stream_ptr) -> None:
+        cam, p, m = self._diff_inputs(camera, params, pmap)
+        self._check(self._lib.mcrt_diff_render_dev(self._h, _ptr(cam), width, height, spp, first_frame,
+                                                   max_depth, _ptr(p), len(p), _ptr(m), out_ptr,
+                                                   stream_ptr))
+
+    def diff_adjoint_dev(self, camera, width, height, spp, first_frame, max_depth, params, pmap,
+                         d_image_ptr, grad_ptr, stream_ptr) -> None:
+        cam, p, m = self._diff_inputs(camera, params, pmap)
+        self._check(self._lib.mcrt_diff_adjoint_dev(self._h, _ptr(cam), width, height, spp,
+                                                    first_frame, max_depth, _ptr(p), len(p), _ptr(m),
+                                                    d_image_ptr, grad_ptr, stream_ptr))
 
     def set_profiling(self, enabled: bool) -> None:
         self._check(self._lib.mcrt_set_profiling(self._h, 1 if enabled else 0))

@@ -1,0 +1,203 @@
+// adjoint_kernels.hip — differentiable render and path-replay adjoint
+// (gfx950).  No counterpart in the reference; the estimator is the reference
+// path tracer (path_trace.wgsl:583-662, BSDF sampling, no NEE) with one
+// extension: up to kMaxParams "parameter materials", each a Lambertian with
+// RGB albedo a_p and a white emission E_p added at every vertex that hits it.
+// With E_p = 0 a parameter material is exactly the reference's Lambertian.
+//
+// For one path with vertices k = 0..n-1 the radiance is
+//   L = sum_k T_k * E_k + T_n * L_end,   T_0 = 1,  T_{k+1} = T_k * att_k,
+// and the outgoing radiance R_k = E_k + att_k * R_{k+1} obeys R_0 = L.
+// Direction sampling never depends on a_p or E_p, so each sample's path is
+// fixed by its PCG stream and
+//   dL/dE_p    = sum over vertices on p of T_k                 (white)
+//   dL/da_p[c] = sum over vertices on p of T_k[c] * R_{k+1}[c].
+//
+// Path replay backpropagation (Vicini, Speierer, Jakob, "Path Replay
+// Backpropagation", ACM TOG 40(4), 2021): the adjoint kernel traces each
+// sample once to get L, then replays the same PCG stream, recovering
+// R_{k+1} = (R_k - E_k) / att_k on the way out instead of storing the path.
+//
+// Layout: one thread per pixel, frames looped inside the thread.  Gradients
+// sit in registers (kMaxParams x 4), are summed per block in a fixed order
+// and then over blocks in a fixed order, so results are reproducible.
+#include "kernels.h"
+#include "shading.h"
+
+namespace mcrt {
+
+// Evaluates one vertex of the extended estimator.  Returns false when the
+// path ends here (miss, emissive light, absorbed); `emit` is the radiance
+// added at this vertex, `att` the attenuation to the next one, `param` the
+// parameter index or -1.
+struct Vertex {
+    V3 emit, att, dir, p;
+    int param;
+    bool cont;
+};
+
+template <bool kSpheres>
+__device__ __forceinline__ Vertex diff_vertex(const DiffArgs &a, V3 o, V3 d, uint32_t &rng,
+                                              const Stack &st) {
+    Vertex vx;
+    vx.param = -1;
+    vx.cont = false;
+    vx.att = v3(0.0f, 0.0f, 0.0f);
+    Best h = trace_one<kSpheres>(a.scene, o, d, MCRT_T_MIN, MCRT_T_MAX, st);
+    if (h.prim == kNone) {  // miss: environment (:592-597)
+        vx.emit = sample_env(a.scene, d);
+        return vx;
+    }
+    Surface s = surface(a.scene, o, d, h.t, h.u, h.v, h.prim);
+    vx.p = s.p;
+    const uint32_t mi = s.mat < 63u ? s.mat : 63u;
+    const int p = a.pmap[mi];
+    if (p >= 0) {
+        const float4 q = a.params[p];
+        vx.param = p;
+        vx.emit = v3(q.w, q.w, q.w);
+        vx.dir = cosine_scatter(rng, s.n);  // :521-528
+        vx.att = v3(q.x, q.y, q.z);
+        vx.cont = true;
+        return vx;
+    }
+    const GpuMaterial &m = a.scene.mats[mi];
+    if (m.type_pad[0] == MCRT_MAT_EMISSIVE) {  // :608-610
+        vx.emit = m.ior_pad[0] * ld3(m.albedo_fuzz);
+        return vx;
+    }
+    vx.emit = v3(0.0f, 0.0f, 0.0f);
+    Scattered sc = scatter_material(m, d, s, rng);
+    if (sc.absorbed) return vx;
+    vx.dir = sc.dir;
+    vx.att = sc.att;
+    vx.cont = true;
+    return vx;
+}
+
+template <bool kSpheres>
+__device__ __forceinline__ V3 diff_path_radiance(const DiffArgs &a, V3 o, V3 d, uint32_t rng,
+                                                 const Stack &st) {
+    V3 T = v3(1.0f, 1.0f, 1.0f), L = v3(0.0f, 0.0f, 0.0f);
+    for (uint32_t b = 0; b < a.max_depth; ++b) {
+        Vertex vx = diff_vertex<kSpheres>(a, o, d, rng, st);
+        L = L + T * vx.emit;
+        if (!vx.cont) break;
+        T = T * vx.att;
+        o = vx.p;
+        d = vx.dir;
+    }
+    return L;
+}
+
+__device__ __forceinline__ float safe_div(float x, float y) { return y != 0.0f ? x / y : 0.0f; }
+
+template <int kBlock, bool kSpheres, bool kAdjoint>
+__global__ __launch_bounds__(kBlock) void k_diff(DiffArgs a) {
+    __shared__ uint32_t lds[kShortStack * kBlock];
+    __shared__ float red[kAdjoint ? kMaxParams * 4 * kBlock : 1];
+    const uint32_t pix = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t wh = a.width * a.height;
+    Stack st{lds, a.ovf, a.ovf_stride, threadIdx.x, pix, (uint32_t)kBlock};
+    float g[kMaxParams][4];
+#pragma unroll
+    for (int q = 0; q < kMaxParams; ++q) g[q][0] = g[q][1] = g[q][2] = g[q][3] = 0.0f;
+    if (pix < wh) {
+        const uint32_t x = pix % a.width, y = pix / a.width;
+        V3 sum = v3(0.0f, 0.0f, 0.0f);
+        V3 delta = v3(0.0f, 0.0f, 0.0f);
+        if (kAdjoint) {
+            float4 dI = a.d_image[pix];
+            const float inv = 1.0f / (float)a.spp;  // dI/dL_s of the mean
+            delta = v3(dI.x * inv, dI.y * inv, dI.z * inv);
+        }
+        for (uint32_t i = 0; i < a.spp; ++i) {
+            uint32_t rng0;
+            V3 o, d;
+            camera_ray(a.cam, a.width, a.height, x, y, a.frame0 + i, rng0, o, d);
+            const V3 L = diff_path_radiance<kSpheres>(a, o, d, rng0, st);
+            if (!kAdjoint) {
+                sum = sum + L;
+                continue;
+            }
+            // Replay with the same PCG stream.
+            uint32_t rng = rng0;
+            V3 T = v3(1.0f, 1.0f, 1.0f), R = L;
+            for (uint32_t b = 0; b < a.max_depth; ++b) {
+                Vertex vx = diff_vertex<kSpheres>(a, o, d, rng, st);
+                if (!vx.cont) break;
+                const V3 Rn = v3(safe_div(R.x - vx.emit.x, vx.att.x), safe_div(R.y - vx.emit.y, vx.att.y),
+                                 safe_div(R.z - vx.emit.z, vx.att.z));
+                if (vx.param >= 0) {
+#pragma unroll
+                    for (int q = 0; q < kMaxParams; ++q) {
+                        if (q == vx.param) {
+                            g[q][0] += delta.x * (T.x * Rn.x);
+                            g[q][1] += delta.y * (T.y * Rn.y);
+                            g[q][2] += delta.z * (T.z * Rn.z);
+                            g[q][3] += dot(delta, T);
+                        }
+                    }
+                }
+                T = T * vx.att;
+                R = Rn;
+                o = vx.p;
+                d = vx.dir;
+            }
+        }
+        if (!kAdjoint) {
+            const float inv = 1.0f / (float)a.spp;
+            a.image[pix] = make_float4(sum.x * inv, sum.y * inv, sum.z * inv, 1.0f);
+        }
+    }
+    if (kAdjoint) {
+        // Fixed-order block reduction: thread j sums column j over the block.
+#pragma unroll
+        for (int q = 0; q < kMaxParams; ++q)
+#pragma unroll
+            for (int c = 0; c < 4; ++c) red[(q * 4 + c) * kBlock + threadIdx.x] = g[q][c];
+        __syncthreads();
+        if (threadIdx.x < kMaxParams * 4) {
+            float s = 0.0f;
+            for (int t = 0; t < kBlock; ++t) s += red[threadIdx.x * kBlock + t];
+            a.partial[(size_t)blockIdx.x * (kMaxParams * 4) + threadIdx.x] = s;
+        }
+    }
+}
+
+// Sum of the per-block partial gradients, in block order.
+__global__ void k_diff_sum(const float *partial, uint32_t n_blocks, uint32_t n_out, float *out) {
+    uint32_t j = threadIdx.x;
+    if (j >= n_out) return;
+    float s = 0.0f;
+    for (uint32_t b = 0; b < n_blocks; ++b) s += partial[(size_t)b * (kMaxParams * 4) + j];
+    out[j] = s;
+}
+
+constexpr int kDiffBlock = 256;
+
+hipError_t launch_diff(const DiffArgs &a, bool adjoint, hipStream_t s) {
+    const uint32_t wh = a.width * a.height;
+    const uint32_t grid = (wh + kDiffBlock - 1) / kDiffBlock;
+    const bool sph = a.scene.sroot != kNone;
+    if (adjoint) {
+        if (sph)
+            hipLaunchKernelGGL((k_diff<kDiffBlock, true, true>), dim3(grid), dim3(kDiffBlock), 0, s, a);
+        else
+            hipLaunchKernelGGL((k_diff<kDiffBlock, false, true>), dim3(grid), dim3(kDiffBlock), 0, s, a);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(k_diff_sum, dim3(1), dim3(kMaxParams * 4), 0, s, a.partial, grid,
+                           a.n_params * 4u, a.grad);
+    } else {
+        if (sph)
+            hipLaunchKernelGGL((k_diff<kDiffBlock, true, false>), dim3(grid), dim3(kDiffBlock), 0, s, a);
+        else
+            hipLaunchKernelGGL((k_diff<kDiffBlock, false, false>), dim3(grid), dim3(kDiffBlock), 0, s, a);
+    }
+    return hipGetLastError();
+}
+
+uint32_t diff_grid(uint32_t wh) { return (wh + kDiffBlock - 1) / kDiffBlock; }
+
+}  // namespace mcrt

@@ -58,6 +58,7 @@ struct mcrt_ctx {
     DevBuf host_accum, host_gbuf, host_out, host_accum_in;
     DevBuf trace_o, trace_d, trace_id, trace_t, trace_work;
     DevBuf spatial;
+    DevBuf diff_partial, diff_grad;
     int spatial_radius = -1;
     float spatial_sigma = 0.0f;
     // last render bookkeeping
@@ -327,7 +328,7 @@ extern "C" void mcrt_destroy(mcrt_ctx *c) {
                       &c->env, &c->A[0], &c->A[1], &c->B[0], &c->B[1], &c->C[0], &c->C[1], &c->H,
                       &c->samples, &c->counters, &c->host_accum, &c->host_gbuf, &c->host_out,
                       &c->host_accum_in, &c->trace_o, &c->trace_d, &c->trace_id, &c->trace_t,
-                      &c->trace_work, &c->spatial})
+                      &c->trace_work, &c->spatial, &c->diff_partial, &c->diff_grad})
         release(*b);
     for (Launch &l : c->pending) {
         (void)hipEventDestroy(l.start);
@@ -486,9 +487,8 @@ extern "C" int mcrt_render_dev(mcrt_ctx *c, const CameraUniform *cam, uint32_t w
     uint32_t *counters = (uint32_t *)c->counters.p;
     HIP_TRY(c, launch_fill_u32(counters, (uint32_t)n_counters, 0u, s));
 
-    uint32_t g_ext = 0, g_shade = 0;
+    uint32_t g_ext = 0;
     grid_for(c, 0, &g_ext);
-    grid_for(c, 1, &g_shade);
     uint32_t *ovf = nullptr;
     if ((rc = ensure_overflow(c, g_ext * (uint32_t)trace_block(), &ovf))) return rc;
 
@@ -550,7 +550,7 @@ extern "C" int mcrt_render_dev(mcrt_ctx *c, const CameraUniform *cam, uint32_t w
             sh.max_depth = max_depth;
             {
                 Timed t(c, "shade", s);
-                HIP_TRY(c, launch_shade(sh, g_shade, s));
+                HIP_TRY(c, launch_shade(sh, (np + 255u) / 256u, s));
             }
         }
         AccArgs acc{};
@@ -739,6 +739,134 @@ extern "C" int mcrt_denoise_bilateral(mcrt_ctx *c, const DenoiseParams *params, 
                                     (const float *)c->host_gbuf.p, (float *)c->host_out.p, s);
     if (rc) return rc;
     HIP_TRY(c, hipMemcpyAsync(out, c->host_out.p, bytes, hipMemcpyDeviceToHost, s));
+    HIP_TRY(c, hipStreamSynchronize(s));
+    return MCRT_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Differentiable render and path-replay adjoint
+// ---------------------------------------------------------------------------
+namespace {
+
+int diff_setup(mcrt_ctx *c, const CameraUniform *cam, uint32_t width, uint32_t height, uint32_t spp,
+               uint32_t first_frame, uint32_t max_depth, const float *params, uint32_t n_params,
+               const int32_t *param_of_material, DiffArgs &a) {
+    if (!c->has_scene) return fail(c, MCRT_ERR_NO_SCENE, "diff render called before mcrt_upload_scene");
+    if (!cam || !params || !param_of_material) return fail(c, MCRT_ERR_BAD_ARG, "null argument");
+    if (width == 0 || height == 0 || spp == 0) return fail(c, MCRT_ERR_BAD_ARG, "empty render");
+    if (max_depth < 1 || max_depth > 64) return fail(c, MCRT_ERR_BAD_ARG, "max_depth must be in [1, 64]");
+    if (n_params > (uint32_t)kMaxParams)
+        return fail(c, MCRT_ERR_LIMIT, "at most %d parameter materials (got %u)", kMaxParams, n_params);
+    if ((uint64_t)width * height > (1ull << 26)) return fail(c, MCRT_ERR_LIMIT, "image too large");
+    a = DiffArgs{};
+    a.scene = c->scene;
+    a.cam = *cam;
+    a.width = width;
+    a.height = height;
+    a.spp = spp;
+    a.frame0 = first_frame;
+    a.max_depth = max_depth;
+    a.n_params = n_params;
+    for (uint32_t p = 0; p < n_params; ++p) {
+        for (int k = 0; k < 4; ++k)
+            if (!std::isfinite(params[4 * p + k])) return fail(c, MCRT_ERR_BAD_ARG, "non-finite parameter");
+        a.params[p] = make_float4(params[4 * p], params[4 * p + 1], params[4 * p + 2], params[4 * p + 3]);
+    }
+    for (uint32_t m = 0; m < MCRT_MAX_MATERIALS; ++m) {
+        int32_t q = param_of_material[m];
+        if (q < -1 || q >= (int32_t)n_params)
+            return fail(c, MCRT_ERR_BAD_ARG, "param_of_material[%u] = %d out of range", m, q);
+        a.pmap[m] = (int8_t)q;
+    }
+    const uint32_t grid = diff_grid(width * height);
+    a.ovf_stride = grid * 256u;
+    int rc = ensure_overflow(c, a.ovf_stride, &a.ovf);
+    if (rc) return rc;
+    return MCRT_OK;
+}
+
+}  // namespace
+
+extern "C" int mcrt_diff_render_dev(mcrt_ctx *c, const CameraUniform *cam, uint32_t width,
+                                    uint32_t height, uint32_t spp, uint32_t first_frame,
+                                    uint32_t max_depth, const float *params, uint32_t n_params,
+                                    const int32_t *param_of_material, float *out_image_dev,
+                                    void *stream) {
+    int rc = bind(c);
+    if (rc) return rc;
+    DiffArgs a;
+    if ((rc = diff_setup(c, cam, width, height, spp, first_frame, max_depth, params, n_params,
+                         param_of_material, a)))
+        return rc;
+    if (!out_image_dev) return fail(c, MCRT_ERR_BAD_ARG, "null image pointer");
+    a.image = (float4 *)out_image_dev;
+    hipStream_t s = (hipStream_t)stream;
+    c->last_stream = s;
+    Timed t(c, "diff_primal", s);
+    HIP_TRY(c, launch_diff(a, false, s));
+    return MCRT_OK;
+}
+
+extern "C" int mcrt_diff_adjoint_dev(mcrt_ctx *c, const CameraUniform *cam, uint32_t width,
+                                     uint32_t height, uint32_t spp, uint32_t first_frame,
+                                     uint32_t max_depth, const float *params, uint32_t n_params,
+                                     const int32_t *param_of_material, const float *d_image_dev,
+                                     float *out_grad_dev, void *stream) {
+    int rc = bind(c);
+    if (rc) return rc;
+    DiffArgs a;
+    if ((rc = diff_setup(c, cam, width, height, spp, first_frame, max_depth, params, n_params,
+                         param_of_material, a)))
+        return rc;
+    if (!d_image_dev || !out_grad_dev) return fail(c, MCRT_ERR_BAD_ARG, "null pointer");
+    const uint32_t grid = diff_grid(width * height);
+    if ((rc = ensure(c, c->diff_partial, (size_t)grid * kMaxParams * 4 * sizeof(float)))) return rc;
+    a.d_image = (const float4 *)d_image_dev;
+    a.partial = (float *)c->diff_partial.p;
+    a.grad = out_grad_dev;
+    hipStream_t s = (hipStream_t)stream;
+    c->last_stream = s;
+    Timed t(c, "diff_adjoint", s);
+    HIP_TRY(c, launch_diff(a, true, s));
+    return MCRT_OK;
+}
+
+extern "C" int mcrt_diff_render(mcrt_ctx *c, const CameraUniform *cam, uint32_t width, uint32_t height,
+                                uint32_t spp, uint32_t first_frame, uint32_t max_depth,
+                                const float *params, uint32_t n_params,
+                                const int32_t *param_of_material, float *out_image) {
+    int rc = bind(c);
+    if (rc) return rc;
+    if (!out_image || width == 0 || height == 0) return fail(c, MCRT_ERR_BAD_ARG, "bad image");
+    size_t bytes = (size_t)width * height * 16;
+    if ((rc = ensure(c, c->host_out, bytes))) return rc;
+    hipStream_t s = c->own_stream;
+    rc = mcrt_diff_render_dev(c, cam, width, height, spp, first_frame, max_depth, params, n_params,
+                              param_of_material, (float *)c->host_out.p, s);
+    if (rc) return rc;
+    HIP_TRY(c, hipMemcpyAsync(out_image, c->host_out.p, bytes, hipMemcpyDeviceToHost, s));
+    HIP_TRY(c, hipStreamSynchronize(s));
+    return MCRT_OK;
+}
+
+extern "C" int mcrt_diff_adjoint(mcrt_ctx *c, const CameraUniform *cam, uint32_t width,
+                                 uint32_t height, uint32_t spp, uint32_t first_frame,
+                                 uint32_t max_depth, const float *params, uint32_t n_params,
+                                 const int32_t *param_of_material, const float *d_image,
+                                 float *out_grad) {
+    int rc = bind(c);
+    if (rc) return rc;
+    if (!d_image || !out_grad || width == 0 || height == 0) return fail(c, MCRT_ERR_BAD_ARG, "bad args");
+    size_t bytes = (size_t)width * height * 16;
+    if ((rc = ensure(c, c->host_accum_in, bytes)) || (rc = ensure(c, c->diff_grad, 4 * kMaxParams * 4)))
+        return rc;
+    hipStream_t s = c->own_stream;
+    HIP_TRY(c, hipMemcpyAsync(c->host_accum_in.p, d_image, bytes, hipMemcpyHostToDevice, s));
+    rc = mcrt_diff_adjoint_dev(c, cam, width, height, spp, first_frame, max_depth, params, n_params,
+                               param_of_material, (const float *)c->host_accum_in.p,
+                               (float *)c->diff_grad.p, s);
+    if (rc) return rc;
+    HIP_TRY(c, hipMemcpyAsync(out_grad, c->diff_grad.p, (size_t)n_params * 16, hipMemcpyDeviceToHost, s));
     HIP_TRY(c, hipStreamSynchronize(s));
     return MCRT_OK;
 }

@@ -323,6 +323,45 @@ __device__ __forceinline__ void sphere_leaf(const DevScene &sc, uint32_t i, V3 o
     }
 }
 
+// Closest hit of one ray by one thread (world_hit semantics and visit rule of
+// trace_persistent), for megakernels that keep a whole path in one thread.
+template <bool kSpheres>
+__device__ __forceinline__ Best trace_one(const DevScene &sc, V3 o, V3 d, float t_min, float t_max,
+                                          const Stack &st) {
+    const V3 inv = v3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+    Best sb;
+    sb.t = t_max;
+    sb.id = sb.prim = kNone;
+    sb.u = sb.v = 0.0f;
+    uint32_t sp = 0;
+    if (kSpheres && sc.sroot != kNone) {
+        uint32_t cur = enter_root(sc, true, o, inv, t_min, t_max);
+        while (cur != kNone) {
+            if (!(cur & kLeafBit)) {
+                node_step(sc.snodes, o, inv, t_min, t_max, sb.t, cur, sp, st);
+            } else {
+                sphere_leaf(sc, cur & ~kLeafBit, o, d, t_min, t_max, sb);
+                cur = sp ? st.get(--sp) : kNone;
+            }
+        }
+    }
+    const float t_mesh = sb.prim != kNone ? sb.t : t_max;
+    Best mb;
+    mb.t = t_mesh;
+    mb.id = mb.prim = kNone;
+    mb.u = mb.v = 0.0f;
+    uint32_t cur = enter_root(sc, false, o, inv, t_min, t_mesh);
+    while (cur != kNone) {
+        if (!(cur & kLeafBit)) {
+            node_step(sc.mnodes, o, inv, t_min, t_mesh, mb.t, cur, sp, st);
+        } else {
+            tri_leaf(sc, cur & ~kLeafBit, o, d, t_min, t_mesh, mb);
+            cur = sp ? st.get(--sp) : kNone;
+        }
+    }
+    return mb.prim != kNone ? mb : sb;
+}
+
 // IO policy: count(), work(), load(slot, o, d, t_min, t_max), store(slot, best).
 // kSpheres = false compiles the variant for scenes without spheres (no
 // sphere phase state), launched when the sphere BVH is empty.

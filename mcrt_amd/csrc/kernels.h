@@ -69,6 +69,25 @@ struct BilateralArgs {
     const float *spatial;  // (2R+1)^2 weights exp(-r^2/(2 sigma_s^2)), row-major
 };
 
+constexpr int kMaxParams = 8;
+
+struct DiffArgs {
+    DevScene scene;
+    CameraUniform cam;
+    uint32_t width, height, spp, frame0, max_depth, n_params;
+    float4 params[kMaxParams];  // (albedo.rgb, emission)
+    int8_t pmap[MCRT_MAX_MATERIALS];  // material slot -> parameter index or -1
+    float4 *image;           // primal: W*H mean radiance
+    const float4 *d_image;   // adjoint: dLoss/dimage
+    float *partial;          // adjoint: per-block sums, kMaxParams*4 per block
+    float *grad;             // adjoint: n_params*4
+    uint32_t *ovf;
+    uint32_t ovf_stride;
+};
+
+hipError_t launch_diff(const DiffArgs &a, bool adjoint, hipStream_t s);
+uint32_t diff_grid(uint32_t wh);
+
 hipError_t launch_fill_u32(uint32_t *p, uint32_t n, uint32_t v, hipStream_t s);
 hipError_t launch_generate(const GenArgs &a, hipStream_t s);
 hipError_t launch_extend(const ExtArgs &a, uint32_t grid, hipStream_t s);

@@ -5,17 +5,18 @@
 // wavefront pipeline over a batch of F frames x W*H pixels of paths:
 //
 //   k_generate    camera rays + PCG seeds for every (frame, pixel) path
-//   k_extend      closest hit for every queued path (persistent, wave64
-//                 chunks pulled from an atomic counter, LDS traversal stack)
+//   k_extend      closest hit for every queued path (persistent, wave64,
+//                 slots pulled from an atomic counter, lane refill, LDS
+//                 short stack; dev_common.h)
 //   k_shade       G-buffer, emission/miss termination, material scatter,
-//                 wave-aggregated enqueue of surviving paths
+//                 block-aggregated enqueue of surviving paths
 //   k_accumulate  running mean over the batch's frames in frame order
 //
 // Every path draws from its own PCG state in the shader's order, and the
 // accumulation applies mix(prev, s, 1/(min(f,65535)+1)) frame by frame, so
 // the result does not depend on how paths are scheduled.
-#include "dev_common.h"
 #include "kernels.h"
+#include "shading.h"
 
 namespace mcrt {
 
@@ -25,48 +26,6 @@ namespace mcrt {
 // Hit record H = (t, u, v, prim bits) with prim = kNone on a miss.
 
 // ---------------------------------------------------------------------------
-// Sampling helpers, path_trace.wgsl:210-227
-// ---------------------------------------------------------------------------
-__device__ __forceinline__ V3 random_unit_sphere(uint32_t &rng) {
-    float cos_phi = 1.0f - 2.0f * rand_f32(rng);
-    float sin_phi = sqrtf(fmaxf(0.0f, 1.0f - cos_phi * cos_phi));
-    float theta = kTau * rand_f32(rng);
-    return v3(sin_phi * cosf(theta), sin_phi * sinf(theta), cos_phi);
-}
-
-__device__ __forceinline__ V3 cosine_scatter(uint32_t &rng, V3 n) {
-    V3 s = n + random_unit_sphere(rng);
-    if (dot(s, s) < 1e-6f) return n;
-    return normalize(s);
-}
-
-__device__ __forceinline__ V3 reflect_vec(V3 v, V3 n) { return v - (2.0f * dot(v, n)) * n; }
-
-__device__ __forceinline__ V3 refract_vec(V3 v, V3 n, float eta) {
-    float cos_theta = fminf(dot(-v, n), 1.0f);
-    V3 r_perp = eta * (v + cos_theta * n);
-    V3 r_para = (-sqrtf(fmaxf(0.0f, 1.0f - dot(r_perp, r_perp)))) * n;
-    return r_perp + r_para;
-}
-
-__device__ __forceinline__ float schlick(float cos_theta, float eta) {
-    float r0 = (1.0f - eta) / (1.0f + eta);
-    r0 = r0 * r0;
-    return r0 + (1.0f - r0) * powf(1.0f - cos_theta, 5.0f);
-}
-
-// sample_env_map, path_trace.wgsl:233-242.
-__device__ __forceinline__ V3 sample_env(const DevScene &sc, V3 dir) {
-    float phi = atan2f(dir.z, dir.x);
-    float theta = asinf(fminf(fmaxf(dir.y, -1.0f), 1.0f));
-    float u = phi / kTau + 0.5f;
-    float v = 0.5f - theta / kPi;
-    uint32_t ix = f2u_sat(u * (float)sc.env_w) % sc.env_w;
-    uint32_t iy = f2u_sat(v * (float)sc.env_h) % sc.env_h;
-    return xyz(sc.env[(size_t)iy * sc.env_w + ix]);
-}
-
-// ---------------------------------------------------------------------------
 // k_generate: cs_main's seed, jitter and thin-lens ray (:633-652)
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_generate(GenArgs a) {
@@ -74,33 +33,13 @@ __global__ __launch_bounds__(256) void k_generate(GenArgs a) {
     if (p >= a.n_paths) return;
     uint32_t pix = p % a.wh;
     uint32_t fi = p / a.wh;
-    uint32_t frame = a.frame0 + fi;
     uint32_t x = pix % a.width, y = pix / a.width;
-    uint32_t rng = pcg_hash(pcg_hash(x + y * a.width) ^ (frame * 2654435761u));
-    float u = ((float)x + rand_f32(rng)) / (float)a.width;
-    float v = 1.0f - ((float)y + rand_f32(rng)) / (float)a.height;
-    const CameraUniform &c = a.cam;
-    V3 focal = (ld3(c.lower_left) + u * ld3(c.horizontal)) + v * ld3(c.vertical);
-    // random_in_unit_disk (:223-227): two draws even for a pinhole.
-    float r = sqrtf(rand_f32(rng));
-    float th = kTau * rand_f32(rng);
-    float lx = r * cosf(th), ly = r * sinf(th);
-    V3 lens = lx * ld3(c.defocus_u) + ly * ld3(c.defocus_v);
-    V3 ro = ld3(c.origin) + lens;
-    V3 rd = normalize(focal - ro);
+    uint32_t rng;
+    V3 ro, rd;
+    camera_ray(a.cam, a.width, a.height, x, y, a.frame0 + fi, rng, ro, rd);
     a.A[p] = make_float4(ro.x, ro.y, ro.z, __uint_as_float(p));
     a.B[p] = make_float4(rd.x, rd.y, rd.z, __uint_as_float(rng));
     a.C[p] = make_float4(1.0f, 1.0f, 1.0f, __uint_as_float(0u));
-}
-
-// ---------------------------------------------------------------------------
-// Persistent work distribution: each wave takes 64 consecutive slots per
-// atomic.  All lanes of a wave leave the loop together.
-// ---------------------------------------------------------------------------
-__device__ __forceinline__ uint32_t grab64(uint32_t *work) {
-    uint32_t base = 0;
-    if (__lane_id() == 0) base = atomicAdd(work, 64u);
-    return __shfl(base, 0);
 }
 
 // ---------------------------------------------------------------------------
@@ -130,130 +69,77 @@ __global__ __launch_bounds__(kBlock) void k_extend(ExtArgs a) {
 }
 
 // ---------------------------------------------------------------------------
-// k_shade: one bounce of path_color (:589-617) for every queued path
+// k_shade: one bounce of path_color (:589-617) for every queued path.
+// One thread per queued slot; the grid covers the largest possible queue and
+// blocks past the live count return at once.  Survivors are compacted into
+// the next queue with one atomic per block.
 // ---------------------------------------------------------------------------
-struct Surface {
-    V3 p, n;
-    bool front;
-    uint32_t mat;
-};
-
-__device__ __forceinline__ Surface surface(const DevScene &sc, V3 o, V3 d, float4 h) {
-    Surface s;
-    uint32_t prim = __float_as_uint(h.w);
-    float t = h.x;
-    s.p = o + t * d;
-    if (prim & kLeafBit) {  // sphere, :282-287
-        float4 s0 = sc.sph[2u * (prim & ~kLeafBit)];
-        float4 s1 = sc.sph[2u * (prim & ~kLeafBit) + 1u];
-        V3 c = xyz(s0);
-        float r = s0.w;
-        V3 outn = v3((s.p.x - c.x) / r, (s.p.y - c.y) / r, (s.p.z - c.z) / r);
-        s.front = dot(d, outn) < 0.0f;
-        s.n = s.front ? outn : -outn;
-        s.mat = __float_as_uint(s1.x);
-    } else {  // triangle, :445-458
-        uint32_t mat;
-        V3 n = tri_normal(sc, prim, h.y, h.z, mat);
-        s.front = dot(d, n) < 0.0f;
-        s.n = s.front ? n : -n;
-        s.mat = mat & 0x7FFFFFFFu;
-    }
-    return s;
-}
-
 __global__ __launch_bounds__(256) void k_shade(ShadeArgs a) {
+    __shared__ uint32_t s_wave[4];
+    __shared__ uint32_t s_base;
     const uint32_t count = *a.count_in;
-    for (;;) {
-        uint32_t base = grab64(a.work);
-        if (base >= count) break;
-        uint32_t slot = base + __lane_id();
-        bool active = slot < count;
-        bool alive = false;
-        float4 nA, nB, nC;
-        if (active) {
-            float4 A = a.A[slot], B = a.B[slot], C = a.C[slot], H = a.H[slot];
-            V3 o = xyz(A), d = xyz(B), thr = xyz(C);
-            uint32_t pid = __float_as_uint(A.w);
-            uint32_t rng = __float_as_uint(B.w);
-            uint32_t bounce = __float_as_uint(C.w);
-            bool first = bounce == 0u;
-            uint32_t pix = pid % a.wh;
-            bool gb = first && a.gbuf != nullptr && (pid / a.wh) == a.gbuf_frame_index;
-            V3 radiance = v3(0.0f, 0.0f, 0.0f);
-            bool done = true;
-            if (__float_as_uint(H.w) == kNone) {  // miss, :592-597
-                if (gb) a.gbuf[pix] = make_float4(0.0f, 0.0f, 0.0f, MCRT_T_MAX);
-                radiance = thr * sample_env(a.scene, d);
+    const uint32_t base = blockIdx.x * 256u;
+    if (base >= count) return;
+    const uint32_t slot = base + threadIdx.x;
+    bool alive = false;
+    float4 nA, nB, nC;
+    if (slot < count) {
+        float4 A = a.A[slot], B = a.B[slot], C = a.C[slot], H = a.H[slot];
+        V3 o = xyz(A), d = xyz(B), thr = xyz(C);
+        uint32_t pid = __float_as_uint(A.w);
+        uint32_t rng = __float_as_uint(B.w);
+        uint32_t bounce = __float_as_uint(C.w);
+        float4 *gbuf = nullptr;
+        uint32_t pix = 0;
+        if (bounce == 0u && a.gbuf != nullptr && pid / a.wh == a.gbuf_frame_index) {
+            gbuf = a.gbuf;
+            pix = pid % a.wh;
+        }
+        V3 radiance = v3(0.0f, 0.0f, 0.0f);
+        bool done = true;
+        if (__float_as_uint(H.w) == kNone) {  // miss, :592-597
+            if (gbuf) gbuf[pix] = make_float4(0.0f, 0.0f, 0.0f, MCRT_T_MAX);
+            radiance = thr * sample_env(a.scene, d);
+        } else {
+            Surface s = surface(a.scene, o, d, H.x, H.y, H.z, __float_as_uint(H.w));
+            if (gbuf) gbuf[pix] = make_float4(s.n.x, s.n.y, s.n.z, H.x);  // :599-602
+            const GpuMaterial &m = a.scene.mats[s.mat < 63u ? s.mat : 63u];
+            if (m.type_pad[0] == MCRT_MAT_EMISSIVE) {  // :608-610
+                radiance = m.ior_pad[0] * (thr * ld3(m.albedo_fuzz));
             } else {
-                Surface s = surface(a.scene, o, d, H);
-                if (gb) a.gbuf[pix] = make_float4(s.n.x, s.n.y, s.n.z, H.x);
-                uint32_t mi = s.mat < 63u ? s.mat : 63u;
-                const GpuMaterial &m = a.scene.mats[mi];
-                uint32_t type = m.type_pad[0];
-                V3 albedo = ld3(m.albedo_fuzz);
-                if (type == MCRT_MAT_EMISSIVE) {  // :608-610
-                    radiance = m.ior_pad[0] * (thr * albedo);
-                } else {
-                    V3 dir, att;
-                    bool absorbed = false;
-                    if (type == MCRT_MAT_METAL) {  // :530-541
-                        float fuzz = m.albedo_fuzz[3];
-                        V3 refl = reflect_vec(normalize(d), s.n);
-                        V3 pert = refl + fuzz * random_unit_sphere(rng);
-                        dir = normalize(dot(pert, pert) >= 1e-6f ? pert : refl);
-                        att = albedo;
-                        absorbed = dot(dir, s.n) <= 0.0f;
-                    } else if (type == MCRT_MAT_DIELECTRIC) {  // :543-564
-                        att = v3(1.0f, 1.0f, 1.0f);
-                        float ior = m.ior_pad[0];
-                        float eta = s.front ? 1.0f / ior : ior;
-                        float cos_theta = fminf(dot(-d, s.n), 1.0f);
-                        float sin_theta = sqrtf(fmaxf(0.0f, 1.0f - cos_theta * cos_theta));
-                        bool cannot = eta * sin_theta > 1.0f;
-                        float sc = (!s.front && !cannot)
-                                       ? sqrtf(fmaxf(0.0f, 1.0f - eta * eta * (1.0f - cos_theta * cos_theta)))
-                                       : cos_theta;
-                        float refl = schlick(sc, eta);
-                        if (cannot || refl > rand_f32(rng))
-                            dir = reflect_vec(d, s.n);
-                        else
-                            dir = normalize(refract_vec(d, s.n, eta));
-                    } else {  // lambertian :521-528
-                        dir = cosine_scatter(rng, s.n);
-                        att = albedo;
-                    }
-                    if (!absorbed) {
-                        thr = thr * att;
-                        uint32_t nb = bounce + 1u;
-                        if (nb < a.max_depth) {
-                            done = false;
-                            nA = make_float4(s.p.x, s.p.y, s.p.z, A.w);
-                            nB = make_float4(dir.x, dir.y, dir.z, __uint_as_float(rng));
-                            nC = make_float4(thr.x, thr.y, thr.z, __uint_as_float(nb));
-                        }
+                Scattered sc = scatter_material(m, d, s, rng);
+                if (!sc.absorbed) {
+                    thr = thr * sc.att;
+                    uint32_t nb = bounce + 1u;
+                    if (nb < a.max_depth) {
+                        done = false;
+                        nA = make_float4(s.p.x, s.p.y, s.p.z, A.w);
+                        nB = make_float4(sc.dir.x, sc.dir.y, sc.dir.z, __uint_as_float(rng));
+                        nC = make_float4(thr.x, thr.y, thr.z, __uint_as_float(nb));
                     }
                 }
             }
-            if (done) a.samples[pid] = make_float4(radiance.x, radiance.y, radiance.z, 0.0f);
-            alive = !done;
         }
-        // Wave-aggregated enqueue: one atomic per wave.
-        uint64_t mask = __ballot(alive);
-        uint32_t n_alive = (uint32_t)__popcll(mask);
-        if (n_alive) {
-            uint32_t obase = 0;
-            if (__lane_id() == 0) obase = atomicAdd(a.count_out, n_alive);
-            obase = __shfl(obase, 0);
-            uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
-                                                       __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
-            if (alive) {
-                uint32_t o = obase + below;
-                a.A_out[o] = nA;
-                a.B_out[o] = nB;
-                a.C_out[o] = nC;
-            }
-        }
+        if (done) a.samples[pid] = make_float4(radiance.x, radiance.y, radiance.z, 0.0f);
+        alive = !done;
+    }
+    // Block-aggregated enqueue.
+    const uint64_t mask = __ballot(alive);
+    const uint32_t wave = threadIdx.x >> 6;
+    if (__lane_id() == 0) s_wave[wave] = (uint32_t)__popcll(mask);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t total = s_wave[0] + s_wave[1] + s_wave[2] + s_wave[3];
+        s_base = total ? atomicAdd(a.count_out, total) : 0u;
+    }
+    __syncthreads();
+    if (alive) {
+        uint32_t o = s_base;
+        for (uint32_t w = 0; w < wave; ++w) o += s_wave[w];
+        o += __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+        a.A_out[o] = nA;
+        a.B_out[o] = nB;
+        a.C_out[o] = nC;
     }
 }
 
@@ -331,6 +217,7 @@ hipError_t launch_extend(const ExtArgs &a, uint32_t grid, hipStream_t s) {
     return hipGetLastError();
 }
 
+// `grid` = blocks covering the largest queue of the batch.
 hipError_t launch_shade(const ShadeArgs &a, uint32_t grid, hipStream_t s) {
     hipLaunchKernelGGL(k_shade, dim3(grid), dim3(256), 0, s, a);
     return hipGetLastError();

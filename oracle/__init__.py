@@ -190,6 +190,37 @@ def ray_triangle(o, d, v0, v1, v2, tmin=1e-4, tmax=1e9):
     return out
 
 
+def diff_render64(scene: Scene, camera, width, height, spp, first_frame, max_depth, params, pmap,
+                  threads=None):
+    """FP64 image (H, W, 3) of the extended estimator for params (n, 4)."""
+    L = lib()
+    L.oracle_diff_render64.argtypes = [C.c_void_p, C.c_void_p] + [C.c_uint32] * 5 + [
+        C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_int]
+    cam = np.ascontiguousarray(camera)
+    p = np.ascontiguousarray(params, np.float64).reshape(-1, 4)
+    m = np.ascontiguousarray(pmap, np.int32)
+    out = np.empty((height, width, 3), np.float64)
+    L.oracle_diff_render64(scene.ref(), _p(cam), width, height, spp, first_frame, max_depth, _p(p),
+                           len(p), _p(m), _p(out), threads or default_threads())
+    return out
+
+
+def diff_grad64(scene: Scene, camera, width, height, spp, first_frame, max_depth, params, pmap,
+                d_image, threads=None):
+    """Gradient (n, 4) of sum(d_image * image) w.r.t. params, FP64."""
+    L = lib()
+    L.oracle_diff_grad64.argtypes = [C.c_void_p, C.c_void_p] + [C.c_uint32] * 5 + [
+        C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]
+    cam = np.ascontiguousarray(camera)
+    p = np.ascontiguousarray(params, np.float64).reshape(-1, 4)
+    m = np.ascontiguousarray(pmap, np.int32)
+    dimg = np.ascontiguousarray(np.asarray(d_image)[..., :3], np.float64)
+    g = np.empty((len(p), 4), np.float64)
+    L.oracle_diff_grad64(scene.ref(), _p(cam), width, height, spp, first_frame, max_depth, _p(p),
+                         len(p), _p(m), _p(dimg), _p(g), threads or default_threads())
+    return g
+
+
 def scatter(material, rd, n, front: bool, rng: int):
     """One scatter event; returns (dir, att, absorbed, rng_out)."""
     L = lib()

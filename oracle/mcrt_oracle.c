@@ -710,6 +710,151 @@ void oracle_render(const OracleScene *s, const CameraUniform *cam, uint32_t W, u
 }
 
 /* ------------------------------------------------------------------------ */
+/* Differentiable-render reference (extended estimator, see the header).      */
+/* Path geometry in FP32 exactly as oracle_render; radiance and gradients in  */
+/* FP64.  The gradient is forward mode by counting: a radiance term           */
+/* t = T_k * Le_k contains a_m once per earlier vertex on material m, so      */
+/* dt/da_m[c] = t[c] * n_m / a_m[c]; dL/dE_p sums T_k over vertices on p.     */
+/* This is an independent derivation from the GPU's reverse replay.           */
+/* ------------------------------------------------------------------------ */
+#define ORACLE_MAX_PARAMS 8
+
+typedef struct {
+    const OracleScene *s;
+    const CameraUniform *cam;
+    uint32_t W, H, spp, first, depth, n_params;
+    const double *params;
+    const int32_t *pmap;
+    const double *d_img; /* NULL: primal only */
+    double *img;
+    int tid, nthreads;
+    double grad[ORACLE_MAX_PARAMS * 4];
+} DiffJob;
+
+static void diff_term(DiffJob *j, const double T[3], const double Le[3], const int counts[],
+                      const double delta[3], double L[3]) {
+    double t[3] = {T[0] * Le[0], T[1] * Le[1], T[2] * Le[2]};
+    L[0] += t[0]; L[1] += t[1]; L[2] += t[2];
+    if (!j->d_img) return;
+    for (uint32_t m = 0; m < j->n_params; ++m) {
+        if (!counts[m]) continue;
+        for (int c = 0; c < 3; ++c)
+            j->grad[4 * m + c] += delta[c] * t[c] * (double)counts[m] / j->params[4 * m + c];
+    }
+}
+
+static void *diff_worker(void *arg) {
+    DiffJob *j = (DiffJob *)arg;
+    const OracleScene *s = j->s;
+    const CameraUniform *cam = j->cam;
+    memset(j->grad, 0, sizeof j->grad);
+    GpuMaterial lam;
+    memset(&lam, 0, sizeof lam);
+    lam.albedo_fuzz[0] = lam.albedo_fuzz[1] = lam.albedo_fuzz[2] = 0.5f;
+    for (uint32_t y = (uint32_t)j->tid; y < j->H; y += (uint32_t)j->nthreads) {
+        for (uint32_t x = 0; x < j->W; ++x) {
+            uint64_t pix = (uint64_t)y * j->W + x;
+            double delta[3] = {0, 0, 0};
+            if (j->d_img)
+                for (int c = 0; c < 3; ++c) delta[c] = j->d_img[3 * pix + c] / (double)j->spp;
+            double sum[3] = {0, 0, 0};
+            for (uint32_t i = 0; i < j->spp; ++i) {
+                uint32_t rng = oracle_pixel_seed(x, y, j->W, j->first + i);
+                float u = ((float)x + oracle_rand_f32(&rng)) / (float)j->W;
+                float v = 1.0f - ((float)y + oracle_rand_f32(&rng)) / (float)j->H;
+                v3 focal = add(add(ld3(cam->lower_left), scl(u, ld3(cam->horizontal))),
+                               scl(v, ld3(cam->vertical)));
+                float lx, ly;
+                random_in_unit_disk(&rng, &lx, &ly);
+                v3 lens = add(scl(lx, ld3(cam->defocus_u)), scl(ly, ld3(cam->defocus_v)));
+                v3 o = add(ld3(cam->origin), lens);
+                v3 d = normalize(sub(focal, o));
+                double T[3] = {1, 1, 1}, L[3] = {0, 0, 0};
+                int counts[ORACLE_MAX_PARAMS] = {0};
+                for (uint32_t b = 0; b < j->depth; ++b) {
+                    Hit h = world_hit_ref(s, o, d, MCRT_T_MIN, MCRT_T_MAX, NULL);
+                    if (h.t < 0.0f) {
+                        v3 e = sample_env(s, d);
+                        double Le[3] = {e.x, e.y, e.z};
+                        diff_term(j, T, Le, counts, delta, L);
+                        break;
+                    }
+                    uint32_t mi = h.mat < MAX_MAT_IDX ? h.mat : MAX_MAT_IDX;
+                    int p = j->pmap[mi];
+                    const GpuMaterial *m = &s->mats->materials[mi];
+                    double att[3];
+                    if (p >= 0) {
+                        const double *q = &j->params[4 * p];
+                        double Le[3] = {q[3], q[3], q[3]};
+                        diff_term(j, T, Le, counts, delta, L);
+                        if (j->d_img) j->grad[4 * p + 3] += delta[0] * T[0] + delta[1] * T[1] + delta[2] * T[2];
+                        Scatter sr = scatter(&lam, d, &h, &rng);
+                        att[0] = q[0]; att[1] = q[1]; att[2] = q[2];
+                        counts[p] += 1;
+                        d = sr.dir;
+                    } else if (m->type_pad[0] == MCRT_MAT_EMISSIVE) {
+                        double Le[3] = {(double)(m->ior_pad[0] * m->albedo_fuzz[0]),
+                                        (double)(m->ior_pad[0] * m->albedo_fuzz[1]),
+                                        (double)(m->ior_pad[0] * m->albedo_fuzz[2])};
+                        diff_term(j, T, Le, counts, delta, L);
+                        break;
+                    } else {
+                        Scatter sr = scatter(m, d, &h, &rng);
+                        if (sr.absorbed) break;
+                        att[0] = sr.att.x; att[1] = sr.att.y; att[2] = sr.att.z;
+                        d = sr.dir;
+                    }
+                    T[0] *= att[0]; T[1] *= att[1]; T[2] *= att[2];
+                    o = h.p;
+                }
+                sum[0] += L[0]; sum[1] += L[1]; sum[2] += L[2];
+            }
+            if (j->img)
+                for (int c = 0; c < 3; ++c) j->img[3 * pix + c] = sum[c] / (double)j->spp;
+        }
+    }
+    return NULL;
+}
+
+static void diff_run(const OracleScene *s, const CameraUniform *cam, uint32_t W, uint32_t H,
+                     uint32_t spp, uint32_t first, uint32_t depth, const double *params,
+                     uint32_t n_params, const int32_t *pmap, const double *d_img, double *img,
+                     double *grad, int n_threads) {
+    if (n_threads < 1) n_threads = 1;
+    if (n_params > ORACLE_MAX_PARAMS) n_params = ORACLE_MAX_PARAMS;
+    DiffJob *jobs = (DiffJob *)calloc((size_t)n_threads, sizeof(DiffJob));
+    pthread_t *th = (pthread_t *)calloc((size_t)n_threads, sizeof(pthread_t));
+    for (int t = 0; t < n_threads; ++t) {
+        DiffJob *j = &jobs[t];
+        j->s = s; j->cam = cam; j->W = W; j->H = H; j->spp = spp; j->first = first;
+        j->depth = depth; j->n_params = n_params; j->params = params; j->pmap = pmap;
+        j->d_img = d_img; j->img = img; j->tid = t; j->nthreads = n_threads;
+        pthread_create(&th[t], NULL, diff_worker, j);
+    }
+    if (grad) memset(grad, 0, sizeof(double) * 4 * n_params);
+    for (int t = 0; t < n_threads; ++t) {
+        pthread_join(th[t], NULL);
+        if (grad)
+            for (uint32_t k = 0; k < 4 * n_params; ++k) grad[k] += jobs[t].grad[k];
+    }
+    free(jobs);
+    free(th);
+}
+
+void oracle_diff_render64(const OracleScene *s, const CameraUniform *cam, uint32_t W, uint32_t H,
+                          uint32_t spp, uint32_t first, uint32_t depth, const double *params,
+                          uint32_t n_params, const int32_t *pmap, double *out_img, int n_threads) {
+    diff_run(s, cam, W, H, spp, first, depth, params, n_params, pmap, NULL, out_img, NULL, n_threads);
+}
+
+void oracle_diff_grad64(const OracleScene *s, const CameraUniform *cam, uint32_t W, uint32_t H,
+                        uint32_t spp, uint32_t first, uint32_t depth, const double *params,
+                        uint32_t n_params, const int32_t *pmap, const double *d_img,
+                        double *grad, int n_threads) {
+    diff_run(s, cam, W, H, spp, first, depth, params, n_params, pmap, d_img, NULL, grad, n_threads);
+}
+
+/* ------------------------------------------------------------------------ */
 /* Batched closest-hit                                                        */
 /* ------------------------------------------------------------------------ */
 typedef struct {

@@ -104,6 +104,20 @@ void oracle_denoise(const DenoiseParams *p, uint32_t width, uint32_t height,
                     const float *accum, const float *gbuf, float *out,
                     int n_threads);
 
+/* Differentiable-render reference: the reference estimator with up to 8
+ * parameter materials (Lambertian albedo a_p + white emission E_p added at
+ * each vertex on them), FP64 radiance over the FP32 path geometry of
+ * oracle_render.  params: n_params x (r, g, b, E); pmap: 64 entries.
+ * Images are W*H*3 doubles (mean over spp).  The gradient of
+ * sum(d_img * image) is computed by forward-mode counting. */
+void oracle_diff_render64(const OracleScene *s, const CameraUniform *cam, uint32_t W, uint32_t H,
+                          uint32_t spp, uint32_t first, uint32_t depth, const double *params,
+                          uint32_t n_params, const int32_t *pmap, double *out_img, int n_threads);
+void oracle_diff_grad64(const OracleScene *s, const CameraUniform *cam, uint32_t W, uint32_t H,
+                        uint32_t spp, uint32_t first, uint32_t depth, const double *params,
+                        uint32_t n_params, const int32_t *pmap, const double *d_img,
+                        double *grad, int n_threads);
+
 /* One scatter event (path_trace.wgsl:521-573) for tests. */
 void oracle_scatter(const GpuMaterial *m, const float rd[3], const float n[3], int front,
                     uint32_t *rng, float out_dir[3], float out_att[3], int *absorbed);
