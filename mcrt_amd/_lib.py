@@ -49,6 +49,14 @@ def load() -> C.CDLL:
         return _lib
     if not os.path.exists(LIB_PATH):
         raise McrtError(-4, f"{LIB_PATH} is not built; run __graft_entry__.build() first")
+    # One HIP runtime per process: torch bundles its own libamdhip64 (same
+    # SONAME as ROCm's).  Loading torch first makes the library's dependency
+    # resolve to that already-loaded runtime, so device pointers and streams
+    # from torch tensors are shared with the library's kernels.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = C.CDLL(LIB_PATH)
     sig = {
         "mcrt_create": (C.c_int, [C.POINTER(_vp), C.c_int]),

@@ -141,6 +141,9 @@ def test_autograd_binding_and_adam(ctx, mid):
     g = ctx.diff_adjoint(w.camera, w.width, w.height, w.spp, 0, w.depth, inv.init, inv.pmap,
                          np.ones((w.height, w.width, 4), np.float32))
     np.testing.assert_allclose(p.grad.cpu().numpy(), g, rtol=1e-6, atol=0)
-    params, hist = diff.optimize(ctx, inv, steps=80, lr=0.03)
-    assert hist[-1][0] < 0.2 * hist[0][0]
-    assert hist[-1][1] < hist[0][1]
+    # Target from the same PCG frames: the loss is exactly zero at the hidden
+    # parameters, so Adam must drive the albedo error down.
+    params, hist = diff.optimize(ctx, inv, steps=150, lr=0.03, target_frame=0)
+    print(f"adam: loss {hist[0][0]:.3e} -> {hist[-1][0]:.3e}, max albedo err {hist[0][1]:.3f} -> {hist[-1][1]:.3f}")
+    assert hist[-1][0] < 0.05 * hist[0][0]
+    assert hist[-1][1] < 0.25 * hist[0][1]
