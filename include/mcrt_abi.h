@@ -200,6 +200,16 @@ int mcrt_render_dev(mcrt_ctx *ctx, const CameraUniform *camera,
                     const float *accum_in_dev, float *out_accum_dev,
                     float *out_gbuf_dev, void *stream);
 
+/* mcrt_render_dev plus a first-hit albedo buffer (W*H RGBA32F, last frame;
+ * dielectric = 1, sky = 0).  No counterpart in the reference, whose G-buffer
+ * holds normal and depth only: it is the extra input of the learned
+ * denoiser. */
+int mcrt_render_aux_dev(mcrt_ctx *ctx, const CameraUniform *camera,
+                        uint32_t width, uint32_t height, uint32_t spp,
+                        uint32_t first_frame, uint32_t max_depth,
+                        const float *accum_in_dev, float *out_accum_dev,
+                        float *out_gbuf_dev, float *out_albedo_dev, void *stream);
+
 /* Replaces cs_denoise (src/shaders/denoise.wgsl:52-120) as dispatched by
  * src/gpu.rs:1150-1169.  accum/gbuf/out: W*H RGBA32F.  params == NULL uses
  * the reference defaults (src/gpu.rs:177-183). */
@@ -275,6 +285,37 @@ int mcrt_diff_adjoint_dev(mcrt_ctx *ctx, const CameraUniform *camera, uint32_t w
                           uint32_t max_depth, const float *params, uint32_t n_params,
                           const int32_t *param_of_material, const float *d_image_dev,
                           float *out_grad_dev, void *stream);
+
+/* Learned denoiser.  No counterpart in the reference (its only denoiser is
+ * the bilateral filter, which stays the fallback): six 3x3 convolutions
+ * 10 -> 32 x5 -> 3 with ReLU and a global skip in log space, BF16 MFMA
+ * kernels with FP32 accumulation, Adam on FP32 master weights.
+ * Inputs per pixel (RGBA32F images, batch-major): noisy radiance (from
+ * mcrt_render*), first-hit albedo (mcrt_render_aux_dev), G-buffer.
+ * Weights in public order: per layer W[co][ci][ky][kx] then b[co];
+ * mcrt_denoiser_param_count() = 40771. */
+typedef struct mcrt_denoiser mcrt_denoiser;
+int mcrt_denoiser_create(mcrt_ctx *ctx, uint32_t seed, mcrt_denoiser **out);
+void mcrt_denoiser_destroy(mcrt_denoiser *dn);
+uint32_t mcrt_denoiser_param_count(void);
+int mcrt_denoiser_get_weights(mcrt_denoiser *dn, float *out);
+int mcrt_denoiser_set_weights(mcrt_denoiser *dn, const float *in);
+int mcrt_denoiser_infer_dev(mcrt_denoiser *dn, uint32_t batch, uint32_t width, uint32_t height,
+                            const float *noisy_dev, const float *albedo_dev,
+                            const float *gbuf_dev, float *out_dev, void *stream);
+/* One Adam step on the L2 loss between log1p(output) and log1p(ref).
+ * loss_out (host, optional) synchronises the stream. */
+int mcrt_denoiser_train_step_dev(mcrt_denoiser *dn, uint32_t batch, uint32_t width,
+                                 uint32_t height, const float *noisy_dev,
+                                 const float *albedo_dev, const float *gbuf_dev,
+                                 const float *ref_dev, float lr, float *loss_out,
+                                 void *stream);
+/* Loss gradient w.r.t. the public weights for one batch (no update);
+ * device inputs, host outputs; blocks. */
+int mcrt_denoiser_gradient(mcrt_denoiser *dn, uint32_t batch, uint32_t width, uint32_t height,
+                           const float *noisy_dev, const float *albedo_dev,
+                           const float *gbuf_dev, const float *ref_dev, float *out_grad,
+                           float *loss_out);
 
 /* Greedy voxel meshing (src/world/mesher.rs:97-400 as driven by
  * src/gpu.rs:386-458).  Host-only.  `blocks` holds chunks_x * chunks_z chunks

@@ -84,6 +84,15 @@ def load() -> C.CDLL:
         "mcrt_diff_adjoint": (C.c_int, [_vp, _vp, _u32, _u32, _u32, _u32, _u32, _vp, _u32, _vp, _vp, _vp]),
         "mcrt_diff_adjoint_dev": (C.c_int, [_vp, _vp, _u32, _u32, _u32, _u32, _u32, _vp, _u32, _vp, _vp, _vp,
                                             _vp]),
+        "mcrt_render_aux_dev": (C.c_int, [_vp, _vp, _u32, _u32, _u32, _u32, _u32, _vp, _vp, _vp, _vp, _vp]),
+        "mcrt_denoiser_create": (C.c_int, [_vp, _u32, C.POINTER(_vp)]),
+        "mcrt_denoiser_destroy": (None, [_vp]),
+        "mcrt_denoiser_param_count": (_u32, []),
+        "mcrt_denoiser_get_weights": (C.c_int, [_vp, _vp]),
+        "mcrt_denoiser_set_weights": (C.c_int, [_vp, _vp]),
+        "mcrt_denoiser_infer_dev": (C.c_int, [_vp, _u32, _u32, _u32, _vp, _vp, _vp, _vp, _vp]),
+        "mcrt_denoiser_train_step_dev": (C.c_int, [_vp, _u32, _u32, _u32, _vp, _vp, _vp, _vp, _f32, _vp, _vp]),
+        "mcrt_denoiser_gradient": (C.c_int, [_vp, _u32, _u32, _u32, _vp, _vp, _vp, _vp, _vp, _vp]),
         "mcrt_mesh_voxel_chunks": (
             C.c_int,
             [_vp, _u32, _u32, C.c_int32, C.c_int32, _f32, _vp, _vp, _vp, C.c_uint64, _vp, C.c_uint64,
@@ -105,7 +114,10 @@ def exported_symbols() -> list:
         "mcrt_trace_closest", "mcrt_trace_closest_dev", "mcrt_build_mesh_bvh",
         "mcrt_build_sphere_bvh", "mcrt_set_profiling", "mcrt_kernel_stats", "mcrt_reset_stats",
         "mcrt_render_counters", "mcrt_mesh_voxel_chunks", "mcrt_diff_render", "mcrt_diff_render_dev",
-        "mcrt_diff_adjoint", "mcrt_diff_adjoint_dev",
+        "mcrt_diff_adjoint", "mcrt_diff_adjoint_dev", "mcrt_render_aux_dev", "mcrt_denoiser_create",
+        "mcrt_denoiser_destroy", "mcrt_denoiser_param_count", "mcrt_denoiser_get_weights",
+        "mcrt_denoiser_set_weights", "mcrt_denoiser_infer_dev", "mcrt_denoiser_train_step_dev",
+        "mcrt_denoiser_gradient",
     ]
 
 
@@ -314,6 +326,13 @@ class Context:
         self._check(self._lib.mcrt_diff_adjoint_dev(self._h, _ptr(cam), width, height, spp,
                                                     first_frame, max_depth, _ptr(p), len(p), _ptr(m),
                                                     d_image_ptr, grad_ptr, stream_ptr))
+
+    def render_aux_dev(self, camera, width, height, spp, first_frame, max_depth, accum_in_ptr,
+                       out_accum_ptr, out_gbuf_ptr, out_albedo_ptr, stream_ptr) -> None:
+        cam = np.ascontiguousarray(camera, T.CameraUniform)
+        self._check(self._lib.mcrt_render_aux_dev(self._h, _ptr(cam), width, height, spp, first_frame,
+                                                  max_depth, accum_in_ptr, out_accum_ptr, out_gbuf_ptr,
+                                                  out_albedo_ptr, stream_ptr))
 
     def set_profiling(self, enabled: bool) -> None:
         self._check(self._lib.mcrt_set_profiling(self._h, 1 if enabled else 0))

@@ -14,6 +14,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <random>
 #include <string>
 #include <vector>
 
@@ -453,10 +454,25 @@ extern "C" int mcrt_upload_scene(mcrt_ctx *c, const GpuSphere *spheres, uint32_t
     return MCRT_OK;
 }
 
+extern "C" int mcrt_render_aux_dev(mcrt_ctx *c, const CameraUniform *cam, uint32_t width,
+                                   uint32_t height, uint32_t spp, uint32_t first_frame,
+                                   uint32_t max_depth, const float *accum_in_dev,
+                                   float *out_accum_dev, float *out_gbuf_dev,
+                                   float *out_albedo_dev, void *stream);
+
 extern "C" int mcrt_render_dev(mcrt_ctx *c, const CameraUniform *cam, uint32_t width,
                                uint32_t height, uint32_t spp, uint32_t first_frame,
                                uint32_t max_depth, const float *accum_in_dev, float *out_accum_dev,
                                float *out_gbuf_dev, void *stream) {
+    return mcrt_render_aux_dev(c, cam, width, height, spp, first_frame, max_depth, accum_in_dev,
+                               out_accum_dev, out_gbuf_dev, nullptr, stream);
+}
+
+extern "C" int mcrt_render_aux_dev(mcrt_ctx *c, const CameraUniform *cam, uint32_t width,
+                                   uint32_t height, uint32_t spp, uint32_t first_frame,
+                                   uint32_t max_depth, const float *accum_in_dev,
+                                   float *out_accum_dev, float *out_gbuf_dev,
+                                   float *out_albedo_dev, void *stream) {
     int rc = bind(c);
     if (rc) return rc;
     if (!c->has_scene) return fail(c, MCRT_ERR_NO_SCENE, "render called before mcrt_upload_scene");
@@ -545,6 +561,7 @@ extern "C" int mcrt_render_dev(mcrt_ctx *c, const CameraUniform *cam, uint32_t w
             sh.samples = (float4 *)c->samples.p;
             const bool has_last = last_frame >= f0 && last_frame < f0 + nf;
             sh.gbuf = (has_last && b == 0) ? (float4 *)out_gbuf_dev : nullptr;
+            sh.albedo = (has_last && b == 0) ? (float4 *)out_albedo_dev : nullptr;
             sh.gbuf_frame_index = last_frame - f0;
             sh.wh = wh;
             sh.max_depth = max_depth;
@@ -868,6 +885,314 @@ extern "C" int mcrt_diff_adjoint(mcrt_ctx *c, const CameraUniform *cam, uint32_t
     if (rc) return rc;
     HIP_TRY(c, hipMemcpyAsync(out_grad, c->diff_grad.p, (size_t)n_params * 16, hipMemcpyDeviceToHost, s));
     HIP_TRY(c, hipStreamSynchronize(s));
+    return MCRT_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Learned denoiser
+// ---------------------------------------------------------------------------
+namespace {
+constexpr int kCin[kDenoiseLayers] = {10, 32, 32, 32, 32, 32};
+constexpr int kCout[kDenoiseLayers] = {32, 32, 32, 32, 32, 3};
+constexpr uint32_t kWgradBlocks = 512;
+}  // namespace
+
+struct mcrt_denoiser {
+    mcrt_ctx *c = nullptr;
+    DevBuf w[kDenoiseLayers], m[kDenoiseLayers], v[kDenoiseLayers], g[kDenoiseLayers];
+    DevBuf wf[kDenoiseLayers], wb[kDenoiseLayers];
+    DevBuf act[kDenoiseLayers], dA, dB, out_log, partial, loss_partial, loss;
+    size_t cap_px = 0;
+    uint32_t step = 0;
+};
+
+namespace {
+
+uint32_t public_param_count() {
+    uint32_t n = 0;
+    for (int l = 0; l < kDenoiseLayers; ++l) n += kCout[l] * kCin[l] * 9 + kCout[l];
+    return n;
+}
+
+int dn_workspace(mcrt_denoiser *d, size_t px) {
+    mcrt_ctx *c = d->c;
+    if (px <= d->cap_px) return MCRT_OK;
+    int rc;
+    const size_t act_bytes = px * 32 * sizeof(uint16_t);
+    for (int l = 0; l < kDenoiseLayers; ++l)
+        if ((rc = ensure(c, d->act[l], act_bytes))) return rc;
+    if ((rc = ensure(c, d->dA, act_bytes)) || (rc = ensure(c, d->dB, act_bytes)) ||
+        (rc = ensure(c, d->out_log, px * 16)) ||
+        (rc = ensure(c, d->loss_partial, ((px + 255) / 256) * sizeof(float))))
+        return rc;
+    d->cap_px = px;
+    return MCRT_OK;
+}
+
+int dn_check(mcrt_denoiser *d, uint32_t batch, uint32_t width, uint32_t height) {
+    if (!d) return MCRT_ERR_BAD_ARG;
+    int rc = bind(d->c);
+    if (rc) return rc;
+    if (batch == 0 || width == 0 || height == 0) return fail(d->c, MCRT_ERR_BAD_ARG, "empty batch");
+    const uint64_t px = (uint64_t)batch * width * height;
+    if (px > (1ull << 27) || height > 65535 * 16 || batch > 65535)
+        return fail(d->c, MCRT_ERR_LIMIT, "denoiser batch of %llu pixels is too large",
+                    (unsigned long long)px);
+    return dn_workspace(d, (size_t)px);
+}
+
+// Forward pass into d->out_log; activations kept for training.
+int dn_forward(mcrt_denoiser *d, int B, int W, int H, const float *noisy, const float *albedo,
+               const float *gbuf, hipStream_t s) {
+    mcrt_ctx *c = d->c;
+    PackArgs pa{(const float4 *)noisy, (const float4 *)albedo, (const float4 *)gbuf,
+                (__bf16 *)d->act[0].p, B, H, W};
+    HIP_TRY(c, launch_pack(pa, s));
+    for (int l = 0; l < kDenoiseLayers; ++l) {
+        ConvArgs ca{};
+        ca.in = (const __bf16 *)d->act[l].p;
+        ca.w = (const __bf16 *)d->wf[l].p;
+        ca.bias = (const float *)d->w[l].p + 9 * 32 * 32;
+        ca.B = B;
+        ca.H = H;
+        ca.W = W;
+        if (l + 1 < kDenoiseLayers) {
+            ca.out = (__bf16 *)d->act[l + 1].p;
+            ca.mode = 0;
+        } else {
+            ca.out_log = (float4 *)d->out_log.p;
+            ca.noisy = (const float4 *)noisy;
+            ca.mode = 2;
+        }
+        Timed t(c, "denoise_conv", s);
+        HIP_TRY(c, launch_conv3x3(ca, s));
+    }
+    return MCRT_OK;
+}
+
+}  // namespace
+
+extern "C" int mcrt_denoiser_create(mcrt_ctx *c, uint32_t seed, mcrt_denoiser **out) {
+    if (!out) return MCRT_ERR_BAD_ARG;
+    *out = nullptr;
+    int rc = bind(c);
+    if (rc) return rc;
+    mcrt_denoiser *d = new mcrt_denoiser();
+    d->c = c;
+    std::mt19937 gen(seed);
+    for (int l = 0; l < kDenoiseLayers; ++l) {
+        std::vector<float> host(kLayerParams, 0.0f);
+        // He initialisation for the ReLU layers; the last layer starts small
+        // so the network begins close to the identity (the global skip).
+        const float std = l + 1 < kDenoiseLayers ? std::sqrt(2.0f / (9.0f * kCin[l])) : 1e-3f;
+        std::normal_distribution<float> nd(0.0f, std);
+        for (int tap = 0; tap < 9; ++tap)
+            for (int co = 0; co < kCout[l]; ++co)
+                for (int ci = 0; ci < kCin[l]; ++ci) host[tap * 1024 + co * 32 + ci] = nd(gen);
+        std::vector<float> zeros(kLayerParams, 0.0f);
+        if ((rc = upload(c, d->w[l], host.data(), kLayerParams * 4)) ||
+            (rc = upload(c, d->m[l], zeros.data(), kLayerParams * 4)) ||
+            (rc = upload(c, d->v[l], zeros.data(), kLayerParams * 4)) ||
+            (rc = ensure(c, d->g[l], kLayerParams * 4)) || (rc = ensure(c, d->wf[l], 9 * 1024 * 2)) ||
+            (rc = ensure(c, d->wb[l], 9 * 1024 * 2))) {
+            mcrt_denoiser_destroy(d);
+            return rc;
+        }
+        if (launch_weights_to_bf16((const float *)d->w[l].p, d->wf[l].p, d->wb[l].p, nullptr) != hipSuccess) {
+            mcrt_denoiser_destroy(d);
+            return fail(c, MCRT_ERR_HIP, "weight conversion launch failed");
+        }
+    }
+    if (hipDeviceSynchronize() != hipSuccess) {
+        mcrt_denoiser_destroy(d);
+        return fail(c, MCRT_ERR_HIP, "denoiser initialisation failed");
+    }
+    *out = d;
+    return MCRT_OK;
+}
+
+extern "C" void mcrt_denoiser_destroy(mcrt_denoiser *d) {
+    if (!d) return;
+    (void)hipSetDevice(d->c->device);
+    (void)hipDeviceSynchronize();
+    for (int l = 0; l < kDenoiseLayers; ++l)
+        for (DevBuf *b : {&d->w[l], &d->m[l], &d->v[l], &d->g[l], &d->wf[l], &d->wb[l], &d->act[l]})
+            release(*b);
+    for (DevBuf *b : {&d->dA, &d->dB, &d->out_log, &d->partial, &d->loss_partial, &d->loss}) release(*b);
+    delete d;
+}
+
+extern "C" uint32_t mcrt_denoiser_param_count(void) { return public_param_count(); }
+
+extern "C" int mcrt_denoiser_get_weights(mcrt_denoiser *d, float *out) {
+    if (!d || !out) return MCRT_ERR_BAD_ARG;
+    int rc = bind(d->c);
+    if (rc) return rc;
+    HIP_TRY(d->c, hipDeviceSynchronize());
+    std::vector<float> host(kLayerParams);
+    size_t k = 0;
+    for (int l = 0; l < kDenoiseLayers; ++l) {
+        HIP_TRY(d->c, hipMemcpy(host.data(), d->w[l].p, kLayerParams * 4, hipMemcpyDeviceToHost));
+        for (int co = 0; co < kCout[l]; ++co)
+            for (int ci = 0; ci < kCin[l]; ++ci)
+                for (int tap = 0; tap < 9; ++tap) out[k++] = host[tap * 1024 + co * 32 + ci];
+        for (int co = 0; co < kCout[l]; ++co) out[k++] = host[9 * 1024 + co];
+    }
+    return MCRT_OK;
+}
+
+extern "C" int mcrt_denoiser_set_weights(mcrt_denoiser *d, const float *in) {
+    if (!d || !in) return MCRT_ERR_BAD_ARG;
+    int rc = bind(d->c);
+    if (rc) return rc;
+    HIP_TRY(d->c, hipDeviceSynchronize());
+    size_t k = 0;
+    for (int l = 0; l < kDenoiseLayers; ++l) {
+        std::vector<float> host(kLayerParams, 0.0f), zeros(kLayerParams, 0.0f);
+        for (int co = 0; co < kCout[l]; ++co)
+            for (int ci = 0; ci < kCin[l]; ++ci)
+                for (int tap = 0; tap < 9; ++tap) host[tap * 1024 + co * 32 + ci] = in[k++];
+        for (int co = 0; co < kCout[l]; ++co) host[9 * 1024 + co] = in[k++];
+        for (size_t i = 0; i < host.size(); ++i)
+            if (!std::isfinite(host[i])) return fail(d->c, MCRT_ERR_BAD_ARG, "non-finite weight");
+        if ((rc = upload(d->c, d->w[l], host.data(), kLayerParams * 4)) ||
+            (rc = upload(d->c, d->m[l], zeros.data(), kLayerParams * 4)) ||
+            (rc = upload(d->c, d->v[l], zeros.data(), kLayerParams * 4)))
+            return rc;
+        HIP_TRY(d->c, launch_weights_to_bf16((const float *)d->w[l].p, d->wf[l].p, d->wb[l].p, nullptr));
+    }
+    d->step = 0;
+    HIP_TRY(d->c, hipDeviceSynchronize());
+    return MCRT_OK;
+}
+
+extern "C" int mcrt_denoiser_infer_dev(mcrt_denoiser *d, uint32_t batch, uint32_t width,
+                                       uint32_t height, const float *noisy, const float *albedo,
+                                       const float *gbuf, float *out, void *stream) {
+    int rc = dn_check(d, batch, width, height);
+    if (rc) return rc;
+    if (!noisy || !albedo || !gbuf || !out) return fail(d->c, MCRT_ERR_BAD_ARG, "null image pointer");
+    hipStream_t s = (hipStream_t)stream;
+    d->c->last_stream = s;
+    if ((rc = dn_forward(d, (int)batch, (int)width, (int)height, noisy, albedo, gbuf, s))) return rc;
+    HIP_TRY(d->c, launch_unpack((const float4 *)d->out_log.p, (float4 *)out,
+                                (size_t)batch * width * height, s));
+    return MCRT_OK;
+}
+
+extern "C" int mcrt_denoiser_train_step_dev(mcrt_denoiser *d, uint32_t batch, uint32_t width,
+                                            uint32_t height, const float *noisy, const float *albedo,
+                                            const float *gbuf, const float *ref, float lr,
+                                            float *loss_out, void *stream) {
+    int rc = dn_check(d, batch, width, height);
+    if (rc) return rc;
+    mcrt_ctx *c = d->c;
+    if (!noisy || !albedo || !gbuf || !ref) return fail(c, MCRT_ERR_BAD_ARG, "null image pointer");
+    if (!(lr > 0.0f) || !std::isfinite(lr)) return fail(c, MCRT_ERR_BAD_ARG, "learning rate must be positive");
+    hipStream_t s = (hipStream_t)stream;
+    c->last_stream = s;
+    const int B = (int)batch, W = (int)width, H = (int)height;
+    if ((rc = dn_forward(d, B, W, H, noisy, albedo, gbuf, s))) return rc;
+    if ((rc = ensure(c, d->partial, (size_t)kWgradBlocks * kLayerParams * 4)) ||
+        (rc = ensure(c, d->loss, 4)))
+        return rc;
+    LossArgs la{(const float4 *)d->out_log.p, (const float4 *)ref, (__bf16 *)d->dA.p,
+                (float *)d->loss_partial.p, 1.0f / (3.0f * (float)B * H * W), B, H, W};
+    HIP_TRY(c, launch_loss(la, (float *)d->loss.p, s));
+    // Backward: layer 5 (no ReLU), then 4..0 with the ReLU mask of their output.
+    __bf16 *cur = (__bf16 *)d->dA.p, *nxt = (__bf16 *)d->dB.p;
+    for (int l = kDenoiseLayers - 1; l >= 0; --l) {
+        const __bf16 *mask = l + 1 < kDenoiseLayers ? (const __bf16 *)d->act[l + 1].p : nullptr;
+        WgradArgs wa{(const __bf16 *)d->act[l].p, cur, mask, (float *)d->partial.p, B, H, W};
+        {
+            Timed t(c, "denoise_wgrad", s);
+            HIP_TRY(c, launch_wgrad(wa, kWgradBlocks, (float *)d->g[l].p, s));
+        }
+        if (l > 0) {
+            ConvArgs ca{};
+            ca.in = cur;
+            ca.mask = mask;
+            ca.w = (const __bf16 *)d->wb[l].p;
+            ca.out = nxt;
+            ca.B = B;
+            ca.H = H;
+            ca.W = W;
+            ca.mode = 1;
+            {
+                Timed t(c, "denoise_dgrad", s);
+                HIP_TRY(c, launch_conv3x3(ca, s));
+            }
+            std::swap(cur, nxt);
+        }
+    }
+    d->step += 1;
+    const float b1 = 0.9f, b2 = 0.999f;
+    for (int l = 0; l < kDenoiseLayers; ++l) {
+        AdamArgs aa{(float *)d->w[l].p, (float *)d->m[l].p, (float *)d->v[l].p, (const float *)d->g[l].p,
+                    (__bf16 *)d->wf[l].p, (__bf16 *)d->wb[l].p, lr, b1, b2, 1e-8f,
+                    1.0f - std::pow(b1, (float)d->step), 1.0f - std::pow(b2, (float)d->step)};
+        HIP_TRY(c, launch_adam(aa, s));
+    }
+    if (loss_out) {
+        HIP_TRY(c, hipMemcpyAsync(loss_out, d->loss.p, 4, hipMemcpyDeviceToHost, s));
+        HIP_TRY(c, hipStreamSynchronize(s));
+        *loss_out *= la.inv_count;
+    }
+    return MCRT_OK;
+}
+
+// Gradient of the loss w.r.t. the public weights for one batch, without an
+// Adam step (for checking against an autograd reference).
+extern "C" int mcrt_denoiser_gradient(mcrt_denoiser *d, uint32_t batch, uint32_t width,
+                                      uint32_t height, const float *noisy, const float *albedo,
+                                      const float *gbuf, const float *ref, float *out_grad,
+                                      float *loss_out) {
+    int rc = dn_check(d, batch, width, height);
+    if (rc) return rc;
+    mcrt_ctx *c = d->c;
+    if (!noisy || !albedo || !gbuf || !ref || !out_grad) return fail(c, MCRT_ERR_BAD_ARG, "null pointer");
+    hipStream_t s = c->own_stream;
+    const int B = (int)batch, W = (int)width, H = (int)height;
+    if ((rc = dn_forward(d, B, W, H, noisy, albedo, gbuf, s))) return rc;
+    if ((rc = ensure(c, d->partial, (size_t)kWgradBlocks * kLayerParams * 4)) ||
+        (rc = ensure(c, d->loss, 4)))
+        return rc;
+    LossArgs la{(const float4 *)d->out_log.p, (const float4 *)ref, (__bf16 *)d->dA.p,
+                (float *)d->loss_partial.p, 1.0f / (3.0f * (float)B * H * W), B, H, W};
+    HIP_TRY(c, launch_loss(la, (float *)d->loss.p, s));
+    __bf16 *cur = (__bf16 *)d->dA.p, *nxt = (__bf16 *)d->dB.p;
+    for (int l = kDenoiseLayers - 1; l >= 0; --l) {
+        const __bf16 *mask = l + 1 < kDenoiseLayers ? (const __bf16 *)d->act[l + 1].p : nullptr;
+        WgradArgs wa{(const __bf16 *)d->act[l].p, cur, mask, (float *)d->partial.p, B, H, W};
+        HIP_TRY(c, launch_wgrad(wa, kWgradBlocks, (float *)d->g[l].p, s));
+        if (l > 0) {
+            ConvArgs ca{};
+            ca.in = cur;
+            ca.mask = mask;
+            ca.w = (const __bf16 *)d->wb[l].p;
+            ca.out = nxt;
+            ca.B = B;
+            ca.H = H;
+            ca.W = W;
+            ca.mode = 1;
+            HIP_TRY(c, launch_conv3x3(ca, s));
+            std::swap(cur, nxt);
+        }
+    }
+    HIP_TRY(c, hipStreamSynchronize(s));
+    std::vector<float> host(kLayerParams);
+    size_t k = 0;
+    for (int l = 0; l < kDenoiseLayers; ++l) {
+        HIP_TRY(c, hipMemcpy(host.data(), d->g[l].p, kLayerParams * 4, hipMemcpyDeviceToHost));
+        for (int co = 0; co < kCout[l]; ++co)
+            for (int ci = 0; ci < kCin[l]; ++ci)
+                for (int tap = 0; tap < 9; ++tap) out_grad[k++] = host[tap * 1024 + co * 32 + ci];
+        for (int co = 0; co < kCout[l]; ++co) out_grad[k++] = host[9 * 1024 + co];
+    }
+    if (loss_out) {
+        HIP_TRY(c, hipMemcpy(loss_out, d->loss.p, 4, hipMemcpyDeviceToHost));
+        *loss_out *= la.inv_count;
+    }
     return MCRT_OK;
 }
 

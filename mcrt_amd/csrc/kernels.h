@@ -36,6 +36,7 @@ struct ShadeArgs {
     uint32_t *work;
     float4 *samples;  // per path id
     float4 *gbuf;     // may be null
+    float4 *albedo;   // may be null: first-hit albedo of the G-buffer frame
     uint32_t gbuf_frame_index;  // batch-relative frame whose G-buffer is kept
     uint32_t wh;
     uint32_t max_depth;
@@ -87,6 +88,59 @@ struct DiffArgs {
 
 hipError_t launch_diff(const DiffArgs &a, bool adjoint, hipStream_t s);
 uint32_t diff_grid(uint32_t wh);
+
+// --- learned denoiser (learned_kernels.hip) ---------------------------------
+struct PackArgs {
+    const float4 *noisy, *albedo, *gbuf;
+    __bf16 *out;  // [B][H][W][32]
+    int B, H, W;
+};
+
+struct ConvArgs {
+    const __bf16 *in;    // [B][H][W][32]
+    const __bf16 *mask;  // optional: in *= (mask > 0)
+    const __bf16 *w;     // [9][32 co][32 ci]
+    const float *bias;   // optional [32]
+    __bf16 *out;         // modes 0, 1
+    float4 *out_log;     // mode 2
+    const float4 *noisy; // mode 2 residual source
+    int B, H, W;
+    int mode;            // 0: ReLU, BF16 out; 1: linear, BF16 out; 2: final layer
+};
+
+struct WgradArgs {
+    const __bf16 *x;     // layer input
+    const __bf16 *dy;    // gradient w.r.t. the layer output
+    const __bf16 *mask;  // optional: dy *= (mask > 0), mask = layer output
+    float *partial;      // [blocks][9*32*32 + 32]
+    int B, H, W;
+};
+
+struct LossArgs {
+    const float4 *out_log, *ref;
+    __bf16 *grad;        // [B][H][W][32]
+    float *partial;      // per block
+    float inv_count;     // 1 / (B*H*W*3)
+    int B, H, W;
+};
+
+struct AdamArgs {
+    float *w, *m, *v;
+    const float *grad;
+    __bf16 *wf, *wb;
+    float lr, beta1, beta2, eps, bc1, bc2;
+};
+
+constexpr int kDenoiseLayers = 6;
+constexpr int kLayerParams = 9 * 32 * 32 + 32;
+
+hipError_t launch_pack(const PackArgs &a, hipStream_t s);
+hipError_t launch_conv3x3(const ConvArgs &a, hipStream_t s);
+hipError_t launch_wgrad(const WgradArgs &a, uint32_t blocks, float *out_grad, hipStream_t s);
+hipError_t launch_loss(const LossArgs &a, float *loss_out, hipStream_t s);
+hipError_t launch_adam(const AdamArgs &a, hipStream_t s);
+hipError_t launch_weights_to_bf16(const float *w, void *wf, void *wb, hipStream_t s);
+hipError_t launch_unpack(const float4 *out_log, float4 *out, size_t n, hipStream_t s);
 
 hipError_t launch_fill_u32(uint32_t *p, uint32_t n, uint32_t v, hipStream_t s);
 hipError_t launch_generate(const GenArgs &a, hipStream_t s);

@@ -91,19 +91,25 @@ __global__ __launch_bounds__(256) void k_shade(ShadeArgs a) {
         uint32_t bounce = __float_as_uint(C.w);
         float4 *gbuf = nullptr;
         uint32_t pix = 0;
-        if (bounce == 0u && a.gbuf != nullptr && pid / a.wh == a.gbuf_frame_index) {
+        if (bounce == 0u && (a.gbuf != nullptr || a.albedo != nullptr) && pid / a.wh == a.gbuf_frame_index) {
             gbuf = a.gbuf;
             pix = pid % a.wh;
         }
+        const bool aux = gbuf != nullptr || (bounce == 0u && a.albedo != nullptr && pid / a.wh == a.gbuf_frame_index);
         V3 radiance = v3(0.0f, 0.0f, 0.0f);
         bool done = true;
         if (__float_as_uint(H.w) == kNone) {  // miss, :592-597
             if (gbuf) gbuf[pix] = make_float4(0.0f, 0.0f, 0.0f, MCRT_T_MAX);
+            if (aux && a.albedo) a.albedo[pix] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
             radiance = thr * sample_env(a.scene, d);
         } else {
             Surface s = surface(a.scene, o, d, H.x, H.y, H.z, __float_as_uint(H.w));
             if (gbuf) gbuf[pix] = make_float4(s.n.x, s.n.y, s.n.z, H.x);  // :599-602
             const GpuMaterial &m = a.scene.mats[s.mat < 63u ? s.mat : 63u];
+            if (aux && a.albedo) {  // first-hit albedo (dielectric: 1)
+                V3 al = m.type_pad[0] == MCRT_MAT_DIELECTRIC ? v3(1.0f, 1.0f, 1.0f) : ld3(m.albedo_fuzz);
+                a.albedo[pix] = make_float4(al.x, al.y, al.z, 1.0f);
+            }
             if (m.type_pad[0] == MCRT_MAT_EMISSIVE) {  // :608-610
                 radiance = m.ior_pad[0] * (thr * ld3(m.albedo_fuzz));
             } else {

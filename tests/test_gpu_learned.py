@@ -1,0 +1,131 @@
+"""Learned denoiser (BF16 MFMA kernels) against a plain PyTorch FP32 reference.
+
+The reference is F.conv2d in FP32 with the same weights; with
+`emulate_bf16` it rounds the packed input, the weights and every hidden
+activation to BF16 at the points where the kernels store BF16, so the only
+remaining difference is the FP32 accumulation order (tolerance 2e-3 in log
+space).  Gradients: the kernels carry BF16 activation gradients, autograd
+carries FP32 ones, so per-layer gradients agree to a few percent (relative
+L2 <= 5e-2, cosine >= 0.998).
+"""
+import numpy as np
+import pytest
+
+from mcrt_amd import camera as cam_mod, learned
+
+pytestmark = pytest.mark.gpu
+
+
+def torch_reference(w, noisy, albedo, gbuf, emulate_bf16=True):
+    import torch
+    import torch.nn.functional as F
+
+    bf = (lambda t: t.to(torch.bfloat16).float()) if emulate_bf16 else (lambda t: t)
+    L = torch.log1p(torch.clamp(noisy[..., :3], min=0.0))
+    d = torch.clamp(gbuf[..., 3], 0.0, 1e4)
+    x = torch.cat([L, albedo[..., :3], gbuf[..., :3], (torch.log1p(d) * (1.0 / 9.21044))[..., None]], -1)
+    x = bf(x).permute(0, 3, 1, 2)
+    out = None
+    for l, (W, b) in enumerate(w):
+        y = F.conv2d(x, bf(W), b, padding=1)
+        if l < 5:
+            x = bf(F.relu(y))
+        else:
+            out = L.permute(0, 3, 1, 2) + y
+    return out.permute(0, 2, 3, 1)
+
+
+@pytest.fixture(scope="module")
+def batch(ctx):
+    import torch
+
+    dev = torch.device("cuda:0")
+    B, S = 2, 48
+    noisy = torch.empty((B, S, S, 4), device=dev)
+    albedo = torch.empty_like(noisy)
+    gbuf = torch.empty_like(noisy)
+    ref = torch.empty_like(noisy)
+    tmp = torch.empty((S, S, 4), device=dev)
+    s = torch.cuda.current_stream(dev).cuda_stream
+    for i in range(B):
+        loaded, sb = learned.variant_scene(77 + i, "c1" if i == 0 else "c2")
+        cam = cam_mod.camera_for(loaded, S, S)
+        ctx.upload_scene(sb)
+        ctx.render_aux_dev(cam, S, S, 4, 0, 8, None, noisy[i].data_ptr(), gbuf[i].data_ptr(),
+                           albedo[i].data_ptr(), s)
+        ctx.render_dev(cam, S, S, 64, 1 << 20, 8, None, ref[i].data_ptr(), tmp.data_ptr(), s)
+    torch.cuda.synchronize()
+    return noisy, albedo, gbuf, ref
+
+
+def _weights_torch(dn, requires_grad=False):
+    import torch
+
+    out = []
+    for W, b in learned.split_weights(dn.get_weights()):
+        tw = torch.tensor(W, device="cuda:0", requires_grad=requires_grad)
+        tb = torch.tensor(b, device="cuda:0", requires_grad=requires_grad)
+        out.append((tw, tb))
+    return out
+
+
+def test_inference_matches_fp32_reference(ctx, batch):
+    import torch
+
+    noisy, albedo, gbuf, ref = batch
+    dn = learned.Denoiser(ctx, seed=3)
+    # Non-trivial last layer so every layer matters.
+    w = dn.get_weights()
+    rng = np.random.default_rng(0)
+    last = sum(co * ci * 9 + co for co, ci in learned.layer_shapes()[:5])
+    w[last:] = rng.normal(0, 0.05, len(w) - last).astype(np.float32)
+    dn.set_weights(w)
+    out = torch.empty_like(noisy)
+    dn.infer(noisy, albedo, gbuf, out)
+    torch.cuda.synchronize()
+    ref_log = torch_reference(_weights_torch(dn), noisy, albedo, gbuf)
+    got_log = torch.log1p(out[..., :3])
+    err = (got_log - ref_log).abs().max().item()
+    print(f"max |log out - log ref| = {err:.2e}")
+    assert err < 2e-3
+    assert torch.all(out[..., 3] == 1.0)
+    # Without BF16 emulation the FP32 network differs by the BF16 rounding only.
+    full = torch_reference(_weights_torch(dn), noisy, albedo, gbuf, emulate_bf16=False)
+    assert (got_log - full).abs().max().item() < 0.1
+
+
+def test_gradient_matches_autograd(ctx, batch):
+    import torch
+
+    noisy, albedo, gbuf, ref = batch
+    dn = learned.Denoiser(ctx, seed=5)
+    w = dn.get_weights()
+    rng = np.random.default_rng(1)
+    last = sum(co * ci * 9 + co for co, ci in learned.layer_shapes()[:5])
+    w[last:] = rng.normal(0, 0.05, len(w) - last).astype(np.float32)
+    dn.set_weights(w)
+    g, loss = dn.gradient(noisy, albedo, gbuf, ref)
+    tw = _weights_torch(dn, requires_grad=True)
+    out_log = torch_reference(tw, noisy, albedo, gbuf)
+    tl = torch.mean((out_log - torch.log1p(torch.clamp(ref[..., :3], min=0.0))) ** 2)
+    tl.backward()
+    assert abs(loss - tl.item()) <= 1e-3 * abs(tl.item())
+    k = 0
+    for l, (W, b) in enumerate(tw):
+        for t in (W, b):
+            n = t.numel()
+            a = g[k:k + n]
+            r = t.grad.detach().cpu().numpy().ravel()
+            k += n
+            rel = np.linalg.norm(a - r) / max(np.linalg.norm(r), 1e-30)
+            cos = float(a @ r / max(np.linalg.norm(a) * np.linalg.norm(r), 1e-30))
+            print(f"layer {l} {'W' if t is W else 'b'}: rel {rel:.2e} cos {cos:.5f}")
+            assert rel < 5e-2 and cos > 0.998
+
+
+def test_training_reduces_loss(ctx, batch):
+    noisy, albedo, gbuf, ref = batch
+    dn = learned.Denoiser(ctx, seed=9)
+    losses = [dn.train_step(noisy, albedo, gbuf, ref, 2e-3) for _ in range(60)]
+    print(f"train loss {losses[0]:.4e} -> {losses[-1]:.4e}")
+    assert losses[-1] < 0.5 * losses[0]
