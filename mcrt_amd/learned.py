@@ -9,10 +9,24 @@ Training data: seeded variants of the Cornell box and of the displaced
 icosphere scene with a random camera, random materials and a random light
 position, rendered at 256 x 256 (noisy at `spp_in`, reference at `spp_ref`,
 both with the first-hit albedo and G-buffer of the noisy render).
+
+Evaluation (:func:`time_to_rel_mse`): the 1920 x 1080 icosphere frame,
+rendered at increasing spp and denoised by either filter; relMSE is taken
+on fixed 32 x 32 windows against a 65,536-spp FP64 reference of the same
+windows, and the figure of merit is the wall time of render + denoise at the
+first spp whose relMSE is at most 1e-2.
+
+    python -m mcrt_amd.learned train --out weights.npy   # on a GPU
+    python -m mcrt_amd.learned eval --weights weights.npy
 """
 from __future__ import annotations
 
+import argparse
 import ctypes as C
+import json
+import os
+import sys
+import time
 from dataclasses import dataclass
 from typing import List
 
@@ -23,6 +37,11 @@ from . import configs, scene as scene_mod, scenes
 from ._lib import Context, McrtError, MCRT_OK, load
 
 N_PARAMS = 40771
+PKG_DIR = os.path.dirname(os.path.abspath(__file__))
+WEIGHTS_PATH = os.path.join(PKG_DIR, "weights", "denoiser_c4.npy")
+REFERENCE_CROPS = os.path.join(os.path.dirname(PKG_DIR), "tests", "golden", "c4_reference_crops.npz")
+EVAL_W, EVAL_H, EVAL_DEPTH = 1920, 1080, 8
+INFER_FLOP_PER_PIXEL = 81216  # 2 * 9 * (10*32 + 4*32*32 + 32*3) multiply-adds of the six convs
 
 
 class Denoiser:
@@ -116,7 +135,7 @@ def variant_scene(seed: int, kind: str):
     base = scene_mod.load_scene_from_str(text, f"variant-{seed}")
     extra = None
     if kind == "c2":
-        v, t, _ = scenes.displaced_icosphere(n=24, seed=seed, mat_base=base.name_to_idx["m0"], n_mats=8)
+        v, t, _ = scenes.displaced_icosphere(n=59, seed=seed, mat_base=base.name_to_idx["m0"], n_mats=8)
         extra = (v, t)
     loaded = scene_mod.load_scene_from_str(text, f"variant-{seed}", extra_mesh=extra)
     return loaded, scene_mod.build_scene_buffers(loaded)
@@ -130,7 +149,7 @@ class Pairs:
     ref: object
 
 
-def render_pairs(ctx: Context, n_variants: int, size: int = 256, spp_in: int = 4, spp_ref: int = 1024,
+def render_pairs(ctx: Context, n_variants: int, size: int = 256, spp_in: int = 4, spp_ref: int = 4096,
                  depth: int = 8, seed0: int = 1000, log=None) -> Pairs:
     import torch
 
@@ -141,6 +160,7 @@ def render_pairs(ctx: Context, n_variants: int, size: int = 256, spp_in: int = 4
     gbuf = torch.empty_like(noisy)
     ref = torch.empty_like(noisy)
     s = torch.cuda.current_stream(dev).cuda_stream
+    tmp_g = torch.empty((size, size, 4), dtype=torch.float32, device=dev)
     for i in range(n_variants):
         kind = "c1" if i % 2 == 0 else "c2"
         loaded, sb = variant_scene(seed0 + i, kind)
@@ -149,7 +169,6 @@ def render_pairs(ctx: Context, n_variants: int, size: int = 256, spp_in: int = 4
         ctx.render_aux_dev(cam, size, size, spp_in, 0, depth, None, noisy[i].data_ptr(), gbuf[i].data_ptr(),
                            albedo[i].data_ptr(), s)
         # Reference from an independent frame range.
-        tmp_g = torch.empty((size, size, 4), dtype=torch.float32, device=dev)
         ctx.render_dev(cam, size, size, spp_ref, 1 << 20, depth, None, ref[i].data_ptr(), tmp_g.data_ptr(), s)
         if log and (i % 16 == 0 or i == n_variants - 1):
             torch.cuda.synchronize()
@@ -168,14 +187,189 @@ def train(dn: Denoiser, pairs: Pairs, steps: int, batch: int = 16, lr: float = 1
     for it in range(steps):
         idx = torch.as_tensor(rng.choice(n, size=min(batch, n), replace=False), device=pairs.noisy.device)
         lr_t = lr * (0.1 ** (it / max(steps, 1)))
-        loss = dn.train_step(pairs.noisy[idx].contiguous(), pairs.albedo[idx].contiguous(),
-                             pairs.gbuf[idx].contiguous(), pairs.ref[idx].contiguous(), lr_t,
-                             want_loss=(it % 20 == 0 or it == steps - 1))
+        # Augmentation: one of the 8 symmetries of the square tile, shared by
+        # the batch (the G-buffer normals stay world-space features).
+        k, flip = int(rng.integers(4)), bool(rng.integers(2))
+
+        def aug(t):
+            t = t[idx]
+            if flip:
+                t = torch.flip(t, dims=(2,))
+            return torch.rot90(t, k, dims=(1, 2)).contiguous() if k else t.contiguous()
+
+        loss = dn.train_step(aug(pairs.noisy), aug(pairs.albedo), aug(pairs.gbuf), aug(pairs.ref), lr_t,
+                             want_loss=(it % 50 == 0 or it == steps - 1))
         if loss is not None:
             losses.append(loss)
             if log:
                 log(f"train step {it}: loss {loss:.4e}")
     return losses
+
+
+def save_weights(dn: Denoiser, path: str, meta: dict) -> None:
+    os.makedirs(os.path.dirname(os.path.abspath(path)), exist_ok=True)
+    np.save(path, dn.get_weights())
+    with open(os.path.splitext(path)[0] + ".json", "w") as f:
+        json.dump(meta, f, indent=1)
+
+
+def load_trained(ctx: Context, path: str = WEIGHTS_PATH) -> Denoiser:
+    """A denoiser with the committed trained weights (raises if absent)."""
+    w = np.load(path, allow_pickle=False)
+    dn = Denoiser(ctx, seed=0)
+    dn.set_weights(w)
+    return dn
+
+
+# ---------------------------------------------------------------------------
+# Evaluation: wall time to relMSE <= 1e-2 at 1080p
+# ---------------------------------------------------------------------------
+def eval_workload():
+    return configs.c2(width=EVAL_W, height=EVAL_H, spp=1, depth=EVAL_DEPTH)
+
+
+def load_reference_crops(path: str = REFERENCE_CROPS):
+    d = np.load(path, allow_pickle=False)
+    return d["crops"].astype(np.float64), [tuple(int(v) for v in o) for o in d["origins"]], int(d["spp"])
+
+
+def crops_rel_mse(img: np.ndarray, crops: np.ndarray, origins) -> float:
+    c = crops.shape[1]
+    x = np.stack([img[y0:y0 + c, x0:x0 + c, :3] for x0, y0 in origins])
+    return rel_mse(x, crops)
+
+
+def time_to_rel_mse(ctx: Context, wl, method: str, spps, crops, origins, dn: Denoiser = None,
+                    threshold: float = 1e-2, reps: int = 3, log=None):
+    """Sweep spp upward; at each, time render (+ denoise) over `reps` runs
+    (median, inputs resident, scene uploaded) and take relMSE on the
+    reference windows.  Stops at the first spp at or below `threshold`.
+    method: "none", "bilateral" or "learned"."""
+    import torch
+
+    dev = torch.device("cuda:0")
+    W, H = wl.width, wl.height
+    s = torch.cuda.current_stream(dev).cuda_stream
+    accum = torch.empty((H, W, 4), dtype=torch.float32, device=dev)
+    gbuf, albedo, out = torch.empty_like(accum), torch.empty_like(accum), torch.empty_like(accum)
+    def run(spp):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        if method == "learned":
+            ctx.render_aux_dev(wl.camera, W, H, spp, 0, wl.depth, None, accum.data_ptr(), gbuf.data_ptr(),
+                               albedo.data_ptr(), s)
+            dn.infer(accum[None], albedo[None], gbuf[None], out[None], s)
+        else:
+            ctx.render_dev(wl.camera, W, H, spp, 0, wl.depth, None, accum.data_ptr(), gbuf.data_ptr(), s)
+            if method == "bilateral":
+                ctx.denoise_bilateral_dev(None, W, H, accum.data_ptr(), gbuf.data_ptr(), out.data_ptr(), s)
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0
+
+    run(1)  # warm-up: first launches and the per-size buffers
+    rows, hit = [], None
+    for spp in spps:
+        # Above 128 spp one run lasts ~0.5 s or more and varies little.
+        times = [run(spp) for _ in range(reps if spp <= 128 else 1)]
+        img = (accum if method == "none" else out).cpu().numpy()
+        err = crops_rel_mse(img, crops, origins)
+        row = {"spp": spp, "ms": round(float(np.median(times)) * 1e3, 3), "relmse": round(err, 6)}
+        rows.append(row)
+        if log:
+            log(f"{method} spp {spp}: {row['ms']:.2f} ms relMSE {err:.4e}")
+        if err <= threshold:
+            hit = row
+            break
+    return {"reached": hit, "sweep": rows}
+
+
+def time_infer(dn: Denoiser, W: int, H: int, reps: int = 10) -> float:
+    """Median seconds of one learned-denoiser inference on a W x H frame."""
+    import torch
+
+    dev = torch.device("cuda:0")
+    x = torch.rand((1, H, W, 4), device=dev)
+    g = torch.rand((1, H, W, 4), device=dev)
+    out = torch.empty_like(x)
+    s = torch.cuda.current_stream(dev).cuda_stream
+    dn.infer(x, g, g, out, s)
+    ts = []
+    for _ in range(reps):
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        dn.infer(x, g, g, out, s)
+        b.record()
+        b.synchronize()
+        ts.append(a.elapsed_time(b) * 1e-3)
+    return float(np.median(ts))
+
+
+def evaluate(ctx: Context, dn: Denoiser, reps: int = 3, log=None) -> dict:
+    crops, origins, ref_spp = load_reference_crops()
+    wl = eval_workload()
+    ctx.upload_scene(wl.buffers)
+    spps = [1, 2, 4, 8, 16, 32, 64, 128, 256, 512, 1024, 2048]
+    res = {"frame": f"icosphere_c2 {EVAL_W}x{EVAL_H} depth {EVAL_DEPTH}",
+           "reference": f"{len(origins)} windows {crops.shape[1]}x{crops.shape[2]}, {ref_spp} spp FP64 oracle",
+           "threshold": 1e-2}
+    for method in ("learned", "bilateral", "none"):
+        res[method] = time_to_rel_mse(ctx, wl, method, spps, crops, origins, dn=dn, reps=reps, log=log)
+    t_inf = time_infer(dn, EVAL_W, EVAL_H)
+    res["infer_ms"] = round(t_inf * 1e3, 3)
+    res["infer_tflops"] = round(INFER_FLOP_PER_PIXEL * EVAL_W * EVAL_H / t_inf / 1e12, 2)
+    lt = res["learned"]["reached"]
+    bt = res["bilateral"]["reached"]
+    res["speedup_vs_bilateral"] = (round(bt["ms"] / lt["ms"], 3) if lt and bt else None)
+    return res
+
+
+def _main(argv=None):
+    import torch
+
+    ap = argparse.ArgumentParser(prog="python -m mcrt_amd.learned")
+    sub = ap.add_subparsers(dest="cmd", required=True)
+    t = sub.add_parser("train")
+    t.add_argument("--variants", type=int, default=512)
+    t.add_argument("--spp-in", type=int, default=4)
+    t.add_argument("--spp-ref", type=int, default=4096)
+    t.add_argument("--steps", type=int, default=4000)
+    t.add_argument("--batch", type=int, default=16)
+    t.add_argument("--lr", type=float, default=1e-3)
+    t.add_argument("--seed", type=int, default=0)
+    t.add_argument("--out", default=WEIGHTS_PATH)
+    e = sub.add_parser("eval")
+    e.add_argument("--weights", default=WEIGHTS_PATH)
+    e.add_argument("--reps", type=int, default=3)
+    a = ap.parse_args(argv)
+
+    def log(msg):
+        print(msg, flush=True)
+
+    torch.cuda.init()
+    with Context(0) as ctx:
+        if a.cmd == "train":
+            t0 = time.time()
+            pairs = render_pairs(ctx, a.variants, spp_in=a.spp_in, spp_ref=a.spp_ref, log=log)
+            t_pairs = time.time() - t0
+            log(f"pairs rendered in {t_pairs:.1f} s")
+            dn = Denoiser(ctx, seed=a.seed)
+            t1 = time.time()
+            losses = train(dn, pairs, a.steps, batch=a.batch, lr=a.lr, seed=a.seed, log=log)
+            torch.cuda.synchronize()
+            t_train = time.time() - t1
+            log(f"trained {a.steps} steps in {t_train:.1f} s")
+            save_weights(dn, a.out, {"variants": a.variants, "size": 256, "spp_in": a.spp_in,
+                                     "spp_ref": a.spp_ref, "steps": a.steps, "batch": a.batch, "lr": a.lr,
+                                     "seed": a.seed, "loss_first": losses[0], "loss_last": losses[-1],
+                                     "pairs_s": round(t_pairs, 1), "train_s": round(t_train, 1)})
+            log(f"weights -> {a.out}")
+        else:
+            dn = load_trained(ctx, a.weights)
+            print(json.dumps(evaluate(ctx, dn, reps=a.reps, log=log)), flush=True)
+
+
+if __name__ == "__main__":
+    sys.exit(_main())
 
 
 def rel_mse(x: np.ndarray, y: np.ndarray) -> float:

@@ -221,6 +221,19 @@ def diff_grad64(scene: Scene, camera, width, height, spp, first_frame, max_depth
     return g
 
 
+def reference64_window(scene: Scene, camera, width, height, x0, y0, cw, ch, spp, first_frame,
+                       max_depth, threads=None):
+    """Converged FP64-accumulated reference of a (ch, cw) window, (ch, cw, 3)."""
+    L = lib()
+    L.oracle_reference64_window.argtypes = [C.c_void_p, C.c_void_p] + [C.c_uint32] * 9 + [
+        C.c_void_p, C.c_int]
+    cam = np.ascontiguousarray(camera)
+    out = np.empty((ch, cw, 3), np.float64)
+    L.oracle_reference64_window(scene.ref(), _p(cam), width, height, x0, y0, cw, ch, spp, first_frame,
+                                max_depth, _p(out), threads or default_threads())
+    return out
+
+
 def scatter(material, rd, n, front: bool, rng: int):
     """One scatter event; returns (dir, att, absorbed, rng_out)."""
     L = lib()

@@ -723,6 +723,7 @@ typedef struct {
     const OracleScene *s;
     const CameraUniform *cam;
     uint32_t W, H, spp, first, depth, n_params;
+    uint32_t cx0, cy0, cw, ch; /* pixel window (the whole image by default) */
     const double *params;
     const int32_t *pmap;
     const double *d_img; /* NULL: primal only */
@@ -751,9 +752,9 @@ static void *diff_worker(void *arg) {
     GpuMaterial lam;
     memset(&lam, 0, sizeof lam);
     lam.albedo_fuzz[0] = lam.albedo_fuzz[1] = lam.albedo_fuzz[2] = 0.5f;
-    for (uint32_t y = (uint32_t)j->tid; y < j->H; y += (uint32_t)j->nthreads) {
-        for (uint32_t x = 0; x < j->W; ++x) {
-            uint64_t pix = (uint64_t)y * j->W + x;
+    for (uint32_t y = j->cy0 + (uint32_t)j->tid; y < j->cy0 + j->ch; y += (uint32_t)j->nthreads) {
+        for (uint32_t x = j->cx0; x < j->cx0 + j->cw; ++x) {
+            uint64_t pix = (uint64_t)(y - j->cy0) * j->cw + (x - j->cx0);
             double delta[3] = {0, 0, 0};
             if (j->d_img)
                 for (int c = 0; c < 3; ++c) delta[c] = j->d_img[3 * pix + c] / (double)j->spp;
@@ -816,10 +817,38 @@ static void *diff_worker(void *arg) {
     return NULL;
 }
 
+static void diff_run_window(const OracleScene *s, const CameraUniform *cam, uint32_t W, uint32_t H,
+                            uint32_t cx0, uint32_t cy0, uint32_t cw, uint32_t ch, uint32_t spp,
+                            uint32_t first, uint32_t depth, const double *params,
+                            uint32_t n_params, const int32_t *pmap, const double *d_img,
+                            double *img, double *grad, int n_threads);
+
 static void diff_run(const OracleScene *s, const CameraUniform *cam, uint32_t W, uint32_t H,
                      uint32_t spp, uint32_t first, uint32_t depth, const double *params,
                      uint32_t n_params, const int32_t *pmap, const double *d_img, double *img,
                      double *grad, int n_threads) {
+    diff_run_window(s, cam, W, H, 0, 0, W, H, spp, first, depth, params, n_params, pmap, d_img, img,
+                    grad, n_threads);
+}
+
+/* Converged reference on a pixel window: the reference estimator (no
+ * parameter materials) with FP64 accumulation, mean over spp frames starting
+ * at `first`.  out: ch x cw x 3 doubles. */
+void oracle_reference64_window(const OracleScene *s, const CameraUniform *cam, uint32_t W, uint32_t H,
+                               uint32_t cx0, uint32_t cy0, uint32_t cw, uint32_t ch, uint32_t spp,
+                               uint32_t first, uint32_t depth, double *out, int n_threads) {
+    int32_t pmap[MCRT_MAX_MATERIALS];
+    for (uint32_t m = 0; m < MCRT_MAX_MATERIALS; ++m) pmap[m] = -1;
+    double dummy[4] = {0, 0, 0, 0};
+    diff_run_window(s, cam, W, H, cx0, cy0, cw, ch, spp, first, depth, dummy, 0, pmap, NULL, out, NULL,
+                    n_threads);
+}
+
+static void diff_run_window(const OracleScene *s, const CameraUniform *cam, uint32_t W, uint32_t H,
+                            uint32_t cx0, uint32_t cy0, uint32_t cw, uint32_t ch, uint32_t spp,
+                            uint32_t first, uint32_t depth, const double *params,
+                            uint32_t n_params, const int32_t *pmap, const double *d_img,
+                            double *img, double *grad, int n_threads) {
     if (n_threads < 1) n_threads = 1;
     if (n_params > ORACLE_MAX_PARAMS) n_params = ORACLE_MAX_PARAMS;
     DiffJob *jobs = (DiffJob *)calloc((size_t)n_threads, sizeof(DiffJob));
@@ -827,6 +856,7 @@ static void diff_run(const OracleScene *s, const CameraUniform *cam, uint32_t W,
     for (int t = 0; t < n_threads; ++t) {
         DiffJob *j = &jobs[t];
         j->s = s; j->cam = cam; j->W = W; j->H = H; j->spp = spp; j->first = first;
+        j->cx0 = cx0; j->cy0 = cy0; j->cw = cw; j->ch = ch;
         j->depth = depth; j->n_params = n_params; j->params = params; j->pmap = pmap;
         j->d_img = d_img; j->img = img; j->tid = t; j->nthreads = n_threads;
         pthread_create(&th[t], NULL, diff_worker, j);

@@ -118,6 +118,13 @@ void oracle_diff_grad64(const OracleScene *s, const CameraUniform *cam, uint32_t
                         uint32_t n_params, const int32_t *pmap, const double *d_img,
                         double *grad, int n_threads);
 
+/* Converged reference ("oracle B") on a pixel window of a W x H frame: the
+ * reference estimator with FP64 accumulation of spp frames from `first`.
+ * out: ch * cw * 3 doubles. */
+void oracle_reference64_window(const OracleScene *s, const CameraUniform *cam, uint32_t W, uint32_t H,
+                               uint32_t cx0, uint32_t cy0, uint32_t cw, uint32_t ch, uint32_t spp,
+                               uint32_t first, uint32_t depth, double *out, int n_threads);
+
 /* One scatter event (path_trace.wgsl:521-573) for tests. */
 void oracle_scatter(const GpuMaterial *m, const float rd[3], const float n[3], int front,
                     uint32_t *rng, float out_dir[3], float out_att[3], int *absorbed);

@@ -285,6 +285,21 @@ def test_cornell_render_is_deterministic_and_finite():
     assert np.array_equal(a, b) and np.array_equal(g, h) and np.all(np.isfinite(a))
 
 
+def test_reference_window_is_the_render_mean_on_the_window():
+    # The converged-reference estimator on a window must be the same
+    # per-sample estimator as oracle_render, only accumulated in FP64 over the
+    # window's pixels (a wrong window offset or frame index fails this).  The
+    # render's running mean weights frame f by 1/(f+1), so it is the plain
+    # mean only from frame 0.
+    w = configs.c1(width=40, height=36)
+    sc = oracle.Scene(w.buffers)
+    a, _ = oracle.render(sc, w.camera, 40, 36, 6, first_frame=0, threads=4)
+    win = oracle.reference64_window(sc, w.camera, 40, 36, 7, 5, 20, 13, 6, 0, 8, threads=3)
+    np.testing.assert_allclose(win, a[5:18, 7:27, :3], rtol=2e-6, atol=2e-7)
+    off = oracle.reference64_window(sc, w.camera, 40, 36, 7, 5, 20, 13, 6, 1, 8, threads=3)
+    assert not np.allclose(off, a[5:18, 7:27, :3], rtol=2e-6, atol=2e-7)
+
+
 # ---------------------------------------------------------------------------
 # Bilateral filter invariants (denoise.wgsl:52-120)
 # ---------------------------------------------------------------------------
