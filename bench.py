@@ -14,6 +14,13 @@ stream, and the CPU oracle timed on a bounded sample of the same workload.
 Default workload: the voxel terrain at 1920 x 1080, 16 spp, depth 6, with the
 bilateral denoiser in every step (the 1080p configuration of the metric).
 The 16M-ray closest-hit microbenchmark runs on the displaced-icosphere scene.
+
+Beside the main figure the line carries:
+- ``denoise_eval``: wall time of render + denoise until relMSE <= 1e-2 on the
+  1920 x 1080 icosphere frame, learned (BF16 MFMA) vs bilateral vs none, and
+  the learned denoiser's inference rate against the BF16 MFMA peak;
+- ``inverse``: 200 Adam steps of inverse rendering through the path-replay
+  adjoint (icosphere scene, 512 x 512, 32 spp, depth 6, 8 albedos).
 """
 from __future__ import annotations
 
@@ -28,6 +35,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "Msamples/s at 1080p; time to relMSE≤1e-2, learned vs bilateral denoise"
 HBM_PEAK_GBS = 8000.0
+MFMA_BF16_PEAK_TFLOPS = 2500.0
 
 
 def parse():
@@ -40,6 +48,10 @@ def parse():
     p.add_argument("--no-roofline", action="store_true")
     p.add_argument("--no-microbench", action="store_true")
     p.add_argument("--cpu-seconds", type=float, default=12.0)
+    p.add_argument("--no-denoise-eval", action="store_true",
+                   help="skip the 1080p time-to-relMSE comparison of the learned and bilateral denoisers")
+    p.add_argument("--no-inverse", action="store_true",
+                   help="skip the 200-step inverse-rendering (path-replay adjoint + Adam) timing")
     return p.parse_args()
 
 
@@ -127,6 +139,10 @@ def main():
         mb = configs.c2(width=64, height=64, spp=1)
         ctx.upload_scene(mb.buffers)
         out["microbench"] = microbench(ctx, mb)
+    if not a.no_denoise_eval:
+        out["denoise_eval"] = denoise_eval(ctx)
+    if not a.no_inverse:
+        out["inverse"] = inverse(ctx)
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(w, a.cpu_seconds)
     ctx.close()
@@ -212,6 +228,47 @@ def microbench(ctx, w, n=1 << 24, reps=5):
     return {"rays": n, "mrays_per_s": round(n / el / 1e6, 1),
             "kernel_ms": round(kernel_ms, 3), "kernel_mrays_per_s": round(n / (kernel_ms * 1e-3) / 1e6, 1),
             "hit_fraction": round(hit, 4)}
+
+
+def denoise_eval(ctx):
+    """Wall time of render + denoise to relMSE <= 1e-2 on the 1920 x 1080
+    icosphere frame: learned (committed weights) vs bilateral vs none; relMSE
+    on fixed windows against the 65,536-spp FP64 reference of those windows."""
+    from mcrt_amd import learned
+
+    missing = [p for p in (learned.WEIGHTS_PATH, learned.REFERENCE_CROPS) if not os.path.exists(p)]
+    if missing:
+        return {"skipped": "missing " + ", ".join(os.path.relpath(p, ROOT) for p in missing)}
+    log("denoise eval: 1080p icosphere, learned vs bilateral")
+    dn = learned.load_trained(ctx)
+    res = learned.evaluate(ctx, dn, reps=3, log=log)
+    dn.close()
+    res["infer_mfma"] = {"bound": "mfma", "achieved": res["infer_tflops"], "peak": MFMA_BF16_PEAK_TFLOPS,
+                         "unit": "TFLOP/s", "frac": round(res["infer_tflops"] / MFMA_BF16_PEAK_TFLOPS, 4)}
+    return res
+
+
+def inverse(ctx, steps=200):
+    """Inverse rendering: 200 Adam steps on 8 material albedos through the
+    path-replay adjoint, icosphere scene 512 x 512, 32 spp, depth 6."""
+    import torch
+
+    from mcrt_amd import diff
+
+    log(f"inverse rendering: {steps} Adam steps")
+    inv = diff.inverse_workload()
+    ctx.upload_scene(inv.work.buffers)
+    diff.optimize(ctx, inv, steps=2)  # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    _, hist = diff.optimize(ctx, inv, steps=steps)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    w = inv.work
+    return {"workload": f"{w.name} {w.width}x{w.height} {w.spp} spp depth {w.depth}, {len(inv.init)} params",
+            "steps": steps, "total_s": round(el, 3), "ms_per_step": round(el / steps * 1e3, 3),
+            "loss_first": hist[0][0], "loss_last": hist[-1][0],
+            "max_albedo_err_last": round(hist[-1][1], 4)}
 
 
 def cpu_baseline(w, seconds):
