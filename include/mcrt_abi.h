@@ -303,7 +303,13 @@ int mcrt_denoiser_set_weights(mcrt_denoiser *dn, const float *in);
 int mcrt_denoiser_infer_dev(mcrt_denoiser *dn, uint32_t batch, uint32_t width, uint32_t height,
                             const float *noisy_dev, const float *albedo_dev,
                             const float *gbuf_dev, float *out_dev, void *stream);
-/* One Adam step on the L2 loss between log1p(output) and log1p(ref).
+/* Training loss, mean over pixels and RGB (default LOG_L2):
+ *   LOG_L2:  (log1p(out) - log1p(ref))^2
+ *   REL_MSE: (out - ref)^2 / (ref^2 + 1e-2), the evaluation metric. */
+#define MCRT_DENOISE_LOSS_LOG_L2 0
+#define MCRT_DENOISE_LOSS_REL_MSE 1
+int mcrt_denoiser_set_loss(mcrt_denoiser *dn, int kind);
+/* One Adam step on the selected loss between the output and ref.
  * loss_out (host, optional) synchronises the stream. */
 int mcrt_denoiser_train_step_dev(mcrt_denoiser *dn, uint32_t batch, uint32_t width,
                                  uint32_t height, const float *noisy_dev,

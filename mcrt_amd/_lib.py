@@ -90,6 +90,7 @@ def load() -> C.CDLL:
         "mcrt_denoiser_param_count": (_u32, []),
         "mcrt_denoiser_get_weights": (C.c_int, [_vp, _vp]),
         "mcrt_denoiser_set_weights": (C.c_int, [_vp, _vp]),
+        "mcrt_denoiser_set_loss": (C.c_int, [_vp, C.c_int]),
         "mcrt_denoiser_infer_dev": (C.c_int, [_vp, _u32, _u32, _u32, _vp, _vp, _vp, _vp, _vp]),
         "mcrt_denoiser_train_step_dev": (C.c_int, [_vp, _u32, _u32, _u32, _vp, _vp, _vp, _vp, _f32, _vp, _vp]),
         "mcrt_denoiser_gradient": (C.c_int, [_vp, _u32, _u32, _u32, _vp, _vp, _vp, _vp, _vp, _vp]),
@@ -117,7 +118,7 @@ def exported_symbols() -> list:
         "mcrt_diff_adjoint", "mcrt_diff_adjoint_dev", "mcrt_render_aux_dev", "mcrt_denoiser_create",
         "mcrt_denoiser_destroy", "mcrt_denoiser_param_count", "mcrt_denoiser_get_weights",
         "mcrt_denoiser_set_weights", "mcrt_denoiser_infer_dev", "mcrt_denoiser_train_step_dev",
-        "mcrt_denoiser_gradient",
+        "mcrt_denoiser_gradient", "mcrt_denoiser_set_loss",
     ]
 
 

@@ -904,6 +904,7 @@ struct mcrt_denoiser {
     DevBuf act[kDenoiseLayers], dA, dB, out_log, partial, loss_partial, loss;
     size_t cap_px = 0;
     uint32_t step = 0;
+    int loss_kind = MCRT_DENOISE_LOSS_LOG_L2;
 };
 
 namespace {
@@ -1041,6 +1042,14 @@ extern "C" int mcrt_denoiser_get_weights(mcrt_denoiser *d, float *out) {
     return MCRT_OK;
 }
 
+extern "C" int mcrt_denoiser_set_loss(mcrt_denoiser *d, int kind) {
+    if (!d) return MCRT_ERR_BAD_ARG;
+    if (kind != MCRT_DENOISE_LOSS_LOG_L2 && kind != MCRT_DENOISE_LOSS_REL_MSE)
+        return fail(d->c, MCRT_ERR_BAD_ARG, "unknown denoiser loss kind");
+    d->loss_kind = kind;
+    return MCRT_OK;
+}
+
 extern "C" int mcrt_denoiser_set_weights(mcrt_denoiser *d, const float *in) {
     if (!d || !in) return MCRT_ERR_BAD_ARG;
     int rc = bind(d->c);
@@ -1097,7 +1106,8 @@ extern "C" int mcrt_denoiser_train_step_dev(mcrt_denoiser *d, uint32_t batch, ui
         (rc = ensure(c, d->loss, 4)))
         return rc;
     LossArgs la{(const float4 *)d->out_log.p, (const float4 *)ref, (__bf16 *)d->dA.p,
-                (float *)d->loss_partial.p, 1.0f / (3.0f * (float)B * H * W), B, H, W};
+                (float *)d->loss_partial.p, 1.0f / (3.0f * (float)B * H * W), B, H, W,
+                d->loss_kind};
     HIP_TRY(c, launch_loss(la, (float *)d->loss.p, s));
     // Backward: layer 5 (no ReLU), then 4..0 with the ReLU mask of their output.
     __bf16 *cur = (__bf16 *)d->dA.p, *nxt = (__bf16 *)d->dB.p;
@@ -1158,7 +1168,8 @@ extern "C" int mcrt_denoiser_gradient(mcrt_denoiser *d, uint32_t batch, uint32_t
         (rc = ensure(c, d->loss, 4)))
         return rc;
     LossArgs la{(const float4 *)d->out_log.p, (const float4 *)ref, (__bf16 *)d->dA.p,
-                (float *)d->loss_partial.p, 1.0f / (3.0f * (float)B * H * W), B, H, W};
+                (float *)d->loss_partial.p, 1.0f / (3.0f * (float)B * H * W), B, H, W,
+                d->loss_kind};
     HIP_TRY(c, launch_loss(la, (float *)d->loss.p, s));
     __bf16 *cur = (__bf16 *)d->dA.p, *nxt = (__bf16 *)d->dB.p;
     for (int l = kDenoiseLayers - 1; l >= 0; --l) {

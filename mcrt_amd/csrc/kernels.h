@@ -122,6 +122,7 @@ struct LossArgs {
     float *partial;      // per block
     float inv_count;     // 1 / (B*H*W*3)
     int B, H, W;
+    int kind;            // 0: L2 in log1p space; 1: relMSE, (x - r)^2 / (r^2 + 1e-2)
 };
 
 struct AdamArgs {

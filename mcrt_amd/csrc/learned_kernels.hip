@@ -296,9 +296,26 @@ __global__ void k_sum_partials(const float *partial, int n_blocks, int n, float 
 }
 
 // ---------------------------------------------------------------------------
-// Loss: L2 between out_log and log1p(max(ref, 0)) over RGB; gradient to the
-// final layer's output (BF16 NHWC32, channels 0..2) and per-block sums.
+// Loss over RGB, mean over pixels and channels; gradient with respect to the
+// final layer's output o = log1p(x) (BF16 NHWC32, channels 0..2) and
+// per-block sums.  kind 0: (o - log1p(r))^2.  kind 1: relMSE
+// (x - r)^2 / (r^2 + 1e-2) with x = expm1(o), the reference r held constant,
+// so d/do = 2 (x - r) / (r^2 + 1e-2) * exp(o).
 // ---------------------------------------------------------------------------
+__device__ __forceinline__ float loss_term(int kind, float o, float r, float &g) {
+    r = fmaxf(r, 0.0f);
+    if (kind == 0) {
+        const float d = o - log1pf(r);
+        g = 2.0f * d;
+        return d * d;
+    }
+    const float x = expm1f(fminf(o, 30.0f));
+    const float d = x - r;
+    const float w = 1.0f / (r * r + 1e-2f);
+    g = 2.0f * d * w * (x + 1.0f);
+    return d * d * w;
+}
+
 __global__ __launch_bounds__(256) void k_loss(LossArgs a) {
     __shared__ float s[256];
     const size_t n = (size_t)a.B * a.H * a.W;
@@ -306,15 +323,14 @@ __global__ __launch_bounds__(256) void k_loss(LossArgs a) {
     float l = 0.0f;
     if (p < n) {
         const float4 o = a.out_log[p], r = a.ref[p];
-        const float d0 = o.x - log1pf(fmaxf(r.x, 0.0f));
-        const float d1 = o.y - log1pf(fmaxf(r.y, 0.0f));
-        const float d2 = o.z - log1pf(fmaxf(r.z, 0.0f));
-        l = d0 * d0 + d1 * d1 + d2 * d2;
-        const float sc = 2.0f * a.inv_count;
+        float g0, g1, g2;
+        l = loss_term(a.kind, o.x, r.x, g0) + loss_term(a.kind, o.y, r.y, g1) +
+            loss_term(a.kind, o.z, r.z, g2);
+        const float sc = a.inv_count;
         bf16x8 g = {};
-        g[0] = f2b(sc * d0);
-        g[1] = f2b(sc * d1);
-        g[2] = f2b(sc * d2);
+        g[0] = f2b(sc * g0);
+        g[1] = f2b(sc * g1);
+        g[2] = f2b(sc * g2);
         bf16x8 z = {};
         bf16x8 *o8 = reinterpret_cast<bf16x8 *>(a.grad + p * kC);
         o8[0] = g;

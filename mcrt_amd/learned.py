@@ -69,6 +69,13 @@ class Denoiser:
         self.ctx._check(self._lib.mcrt_denoiser_get_weights(self._h, out.ctypes.data))
         return out
 
+    LOSSES = {"log_l2": 0, "rel_mse": 1}
+
+    def set_loss(self, kind: str) -> None:
+        """Training loss: "log_l2" (L2 of log1p, default) or "rel_mse"
+        ((x - r)^2 / (r^2 + 1e-2), the evaluation metric)."""
+        self.ctx._check(self._lib.mcrt_denoiser_set_loss(self._h, self.LOSSES[kind]))
+
     def set_weights(self, w: np.ndarray) -> None:
         w = np.ascontiguousarray(w, np.float32)
         if w.shape != (N_PARAMS,):
@@ -149,6 +156,20 @@ class Pairs:
     ref: object
 
 
+def reference_render(ctx: Context, cam, W: int, H: int, spp_in: int, spp_ref: int, depth: int, noisy, ref,
+                     tmp_gbuf, stream) -> None:
+    """Mean of frames spp_in .. spp_in+spp_ref-1, independent of the noisy
+    input (frames 0 .. spp_in-1).  The renderer's running mean weights frame
+    f by 1/(f+1), i.e. it is a plain mean only from frame 0, so render frames
+    0 .. spp_in+spp_ref-1 and remove the noisy frames' share:
+    ref = ((spp_in+spp_ref) * mean_all - spp_in * noisy) / spp_ref."""
+    n_all = spp_in + spp_ref
+    if n_all > 65535:
+        raise ValueError("reference range exceeds the renderer's frame-count clamp")
+    ctx.render_dev(cam, W, H, n_all, 0, depth, None, ref.data_ptr(), tmp_gbuf.data_ptr(), stream)
+    ref.mul_(n_all / spp_ref).sub_(noisy, alpha=spp_in / spp_ref)
+
+
 def render_pairs(ctx: Context, n_variants: int, size: int = 256, spp_in: int = 4, spp_ref: int = 4096,
                  depth: int = 8, seed0: int = 1000, log=None) -> Pairs:
     import torch
@@ -168,8 +189,7 @@ def render_pairs(ctx: Context, n_variants: int, size: int = 256, spp_in: int = 4
         ctx.upload_scene(sb)
         ctx.render_aux_dev(cam, size, size, spp_in, 0, depth, None, noisy[i].data_ptr(), gbuf[i].data_ptr(),
                            albedo[i].data_ptr(), s)
-        # Reference from an independent frame range.
-        ctx.render_dev(cam, size, size, spp_ref, 1 << 20, depth, None, ref[i].data_ptr(), tmp_g.data_ptr(), s)
+        reference_render(ctx, cam, size, size, spp_in, spp_ref, depth, noisy[i], ref[i], tmp_g, s)
         if log and (i % 16 == 0 or i == n_variants - 1):
             torch.cuda.synchronize()
             log(f"pairs {i + 1}/{n_variants}")
@@ -178,13 +198,18 @@ def render_pairs(ctx: Context, n_variants: int, size: int = 256, spp_in: int = 4
 
 
 def train(dn: Denoiser, pairs: Pairs, steps: int, batch: int = 16, lr: float = 1e-3, seed: int = 0,
-          log=None) -> List[float]:
+          log=None, loss: str = "log_l2", warm_steps: int = 0) -> List[float]:
+    """Adam steps on random batches; the first `warm_steps` use the log-L2
+    loss, the rest `loss`.  The learning rate decays from lr to lr/10."""
     import torch
 
     n = pairs.noisy.shape[0]
     rng = np.random.default_rng(seed)
     losses = []
+    dn.set_loss("log_l2" if warm_steps > 0 else loss)
     for it in range(steps):
+        if it == warm_steps and warm_steps > 0:
+            dn.set_loss(loss)
         idx = torch.as_tensor(rng.choice(n, size=min(batch, n), replace=False), device=pairs.noisy.device)
         lr_t = lr * (0.1 ** (it / max(steps, 1)))
         # Augmentation: one of the 8 symmetries of the square tile, shared by
@@ -197,12 +222,12 @@ def train(dn: Denoiser, pairs: Pairs, steps: int, batch: int = 16, lr: float = 1
                 t = torch.flip(t, dims=(2,))
             return torch.rot90(t, k, dims=(1, 2)).contiguous() if k else t.contiguous()
 
-        loss = dn.train_step(aug(pairs.noisy), aug(pairs.albedo), aug(pairs.gbuf), aug(pairs.ref), lr_t,
-                             want_loss=(it % 50 == 0 or it == steps - 1))
-        if loss is not None:
-            losses.append(loss)
+        value = dn.train_step(aug(pairs.noisy), aug(pairs.albedo), aug(pairs.gbuf), aug(pairs.ref), lr_t,
+                              want_loss=(it % 50 == 0 or it == steps - 1))
+        if value is not None:
+            losses.append(value)
             if log:
-                log(f"train step {it}: loss {loss:.4e}")
+                log(f"train step {it}: loss {value:.4e}")
     return losses
 
 
@@ -224,6 +249,13 @@ def load_trained(ctx: Context, path: str = WEIGHTS_PATH) -> Denoiser:
 # ---------------------------------------------------------------------------
 # Evaluation: wall time to relMSE <= 1e-2 at 1080p
 # ---------------------------------------------------------------------------
+def rel_mse(x: np.ndarray, y: np.ndarray) -> float:
+    """mean over pixels and RGB of (x - y)^2 / (y^2 + 1e-2)."""
+    x = x[..., :3].astype(np.float64)
+    y = y[..., :3].astype(np.float64)
+    return float(np.mean((x - y) ** 2 / (y * y + 1e-2)))
+
+
 def eval_workload():
     return configs.c2(width=EVAL_W, height=EVAL_H, spp=1, depth=EVAL_DEPTH)
 
@@ -308,7 +340,9 @@ def evaluate(ctx: Context, dn: Denoiser, reps: int = 3, log=None) -> dict:
     crops, origins, ref_spp = load_reference_crops()
     wl = eval_workload()
     ctx.upload_scene(wl.buffers)
-    spps = [1, 2, 4, 8, 16, 32, 64, 128, 256, 512, 1024, 2048]
+    # Steps of at most ~1/3 so that neither filter is charged for a coarse grid.
+    spps = [1, 2, 3, 4, 5, 6, 7, 8, 10, 12, 14, 16, 20, 24, 28, 32, 40, 48, 56, 64, 80, 96, 128, 160, 192,
+            256, 320, 384, 512, 640, 768, 1024, 1280, 1536, 2048]
     res = {"frame": f"icosphere_c2 {EVAL_W}x{EVAL_H} depth {EVAL_DEPTH}",
            "reference": f"{len(origins)} windows {crops.shape[1]}x{crops.shape[2]}, {ref_spp} spp FP64 oracle",
            "threshold": 1e-2}
@@ -336,6 +370,8 @@ def _main(argv=None):
     t.add_argument("--batch", type=int, default=16)
     t.add_argument("--lr", type=float, default=1e-3)
     t.add_argument("--seed", type=int, default=0)
+    t.add_argument("--loss", choices=sorted(Denoiser.LOSSES), default="rel_mse")
+    t.add_argument("--warm-steps", type=int, default=2000, help="log-L2 steps before --loss")
     t.add_argument("--out", default=WEIGHTS_PATH)
     e = sub.add_parser("eval")
     e.add_argument("--weights", default=WEIGHTS_PATH)
@@ -354,13 +390,15 @@ def _main(argv=None):
             log(f"pairs rendered in {t_pairs:.1f} s")
             dn = Denoiser(ctx, seed=a.seed)
             t1 = time.time()
-            losses = train(dn, pairs, a.steps, batch=a.batch, lr=a.lr, seed=a.seed, log=log)
+            losses = train(dn, pairs, a.steps, batch=a.batch, lr=a.lr, seed=a.seed, log=log, loss=a.loss,
+                           warm_steps=a.warm_steps)
             torch.cuda.synchronize()
             t_train = time.time() - t1
             log(f"trained {a.steps} steps in {t_train:.1f} s")
             save_weights(dn, a.out, {"variants": a.variants, "size": 256, "spp_in": a.spp_in,
                                      "spp_ref": a.spp_ref, "steps": a.steps, "batch": a.batch, "lr": a.lr,
-                                     "seed": a.seed, "loss_first": losses[0], "loss_last": losses[-1],
+                                     "seed": a.seed, "loss": a.loss, "warm_steps": a.warm_steps,
+                                     "loss_first": losses[0], "loss_last": losses[-1],
                                      "pairs_s": round(t_pairs, 1), "train_s": round(t_train, 1)})
             log(f"weights -> {a.out}")
         else:
@@ -370,10 +408,3 @@ def _main(argv=None):
 
 if __name__ == "__main__":
     sys.exit(_main())
-
-
-def rel_mse(x: np.ndarray, y: np.ndarray) -> float:
-    """mean over pixels and RGB of (x - y)^2 / (y^2 + 1e-2)."""
-    x = x[..., :3].astype(np.float64)
-    y = y[..., :3].astype(np.float64)
-    return float(np.mean((x - y) ** 2 / (y * y + 1e-2)))
