@@ -36,6 +36,10 @@ sys.path.insert(0, ROOT)
 METRIC = "Msamples/s at 1080p; time to relMSE≤1e-2, learned vs bilateral denoise"
 HBM_PEAK_GBS = 8000.0
 MFMA_BF16_PEAK_TFLOPS = 2500.0
+# Measured bytes between L2 and memory per k_extend launch (FETCH_SIZE x 2 +
+# WRITE_SIZE x 1, factors from a 1 GiB copy in the same rocprofv3 --pmc runs),
+# per workload at its default size; profiles/traffic_extend_voxel_c3_1080p.md.
+EXTEND_TRAFFIC_BYTES = {"voxel_c3": 9.292e9}
 
 
 def parse():
@@ -52,6 +56,9 @@ def parse():
                    help="skip the 1080p time-to-relMSE comparison of the learned and bilateral denoisers")
     p.add_argument("--no-inverse", action="store_true",
                    help="skip the 200-step inverse-rendering (path-replay adjoint + Adam) timing")
+    p.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                   help="first copy MIB MiB device-to-device three times (a plain copy of known size that "
+                        "calibrates the FETCH_SIZE / WRITE_SIZE counters)")
     return p.parse_args()
 
 
@@ -76,6 +83,15 @@ def main():
     w = configs.by_name(a.workload)
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
+    if a.copy_calibration > 0:
+        n = a.copy_calibration << 18  # float32 elements
+        src = torch.ones(n, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"copy calibration: 3 x {a.copy_calibration} MiB read + written")
+        del src, dst
     ctx = Context(0)
     ctx.upload_scene(w.buffers)
     accum = torch.empty((w.height, w.width, 4), dtype=torch.float32, device=dev)
@@ -118,7 +134,7 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
-        "data": "synthetic (seeded procedural scene, random-init nothing)",
+        "data": "synthetic (seeded procedural scenes; no external assets)",
         "config": {
             "workload": w.name,
             "width": w.width,
@@ -174,6 +190,8 @@ def roofline(ctx, w, step, segments):
     b_ray = 24.0 + 8.0 + 32.0 * nodes / seg + 36.0 * tris / seg + 32.0 * sph / seg
     total_bytes = b_ray * segments
     achieved = total_bytes / (ms * 1e-3) / 1e9
+    avg_ms = ms / max(launches, 1)
+    traffic = EXTEND_TRAFFIC_BYTES.get(w.name)
     return {
         "kernel": "extend",
         "bound": "hbm",
@@ -181,8 +199,11 @@ def roofline(ctx, w, step, segments):
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4),
-        "traffic": None,
-        "avg_launch_ms": round(ms / max(launches, 1), 4),
+        "traffic": traffic,
+        "traffic_gbs": round(traffic / (avg_ms * 1e-3) / 1e9, 1) if traffic else None,
+        "traffic_frac": round(traffic / (avg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4) if traffic else None,
+        "algorithmic_bytes_per_launch": round(total_bytes / max(launches, 1)),
+        "avg_launch_ms": round(avg_ms, 4),
         "launches": int(launches),
         "bytes_per_segment": round(b_ray, 1),
         "nodes_per_segment": round(nodes / seg, 2),
@@ -258,17 +279,23 @@ def inverse(ctx, steps=200):
     log(f"inverse rendering: {steps} Adam steps")
     inv = diff.inverse_workload()
     ctx.upload_scene(inv.work.buffers)
-    diff.optimize(ctx, inv, steps=2)  # warm-up
+    w = inv.work
+    target_spp = 8 * w.spp
+    diff.optimize(ctx, inv, steps=2, target_spp=target_spp)  # warm-up
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    _, hist = diff.optimize(ctx, inv, steps=steps)
+    _, hist = diff.optimize(ctx, inv, steps=steps, target_spp=target_spp)
     torch.cuda.synchronize()
     el = time.perf_counter() - t0
-    w = inv.work
-    return {"workload": f"{w.name} {w.width}x{w.height} {w.spp} spp depth {w.depth}, {len(inv.init)} params",
+    return {"workload": f"{w.name} {w.width}x{w.height} {w.spp} spp depth {w.depth}, {len(inv.init)} "
+                        f"materials x (RGB albedo, emission); target {target_spp} spp, independent frames; "
+                        "decorrelated primal / replay frames per step",
             "steps": steps, "total_s": round(el, 3), "ms_per_step": round(el / steps * 1e3, 3),
-            "loss_first": hist[0][0], "loss_last": hist[-1][0],
-            "max_albedo_err_last": round(hist[-1][1], 4)}
+            # samples of the primal render plus samples of the adjoint (each traced and replayed)
+            "msamples_fwd_plus_adj_per_s": round(2 * w.width * w.height * w.spp * steps / el / 1e6, 1),
+            "loss_first": round(hist[0][0], 6), "loss_last": round(hist[-1][0], 6),
+            "albedo_err_first": {"max": round(hist[0][1], 4), "mean": round(hist[0][2], 4)},
+            "albedo_err_last": {"max": round(hist[-1][1], 4), "mean": round(hist[-1][2], 4)}}
 
 
 def cpu_baseline(w, seconds):

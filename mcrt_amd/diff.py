@@ -50,26 +50,26 @@ def _autograd_function():
         """image = render(params); params (8, 4) float32 on the GPU."""
 
         @staticmethod
-        def forward(fctx, params, ctx, work, pmap, first_frame):
+        def forward(fctx, params, ctx, work, pmap, first_frame, grad_frame):
             p = params.detach().float().cpu().numpy()
             img = torch.empty((work.height, work.width, 4), dtype=torch.float32, device=params.device)
             s = torch.cuda.current_stream(params.device).cuda_stream
             ctx.diff_render_dev(work.camera, work.width, work.height, work.spp, first_frame,
                                 work.depth, p, pmap, img.data_ptr(), s)
             fctx.save_for_backward(params)
-            fctx.meta = (ctx, work, pmap, first_frame, p)
+            fctx.meta = (ctx, work, pmap, first_frame if grad_frame is None else grad_frame, p)
             return img
 
         @staticmethod
         def backward(fctx, grad_img):
             (params,) = fctx.saved_tensors
-            ctx, work, pmap, first_frame, p = fctx.meta
+            ctx, work, pmap, grad_frame, p = fctx.meta
             g = torch.empty((len(p), 4), dtype=torch.float32, device=params.device)
             gi = grad_img.contiguous().float()
             s = torch.cuda.current_stream(params.device).cuda_stream
-            ctx.diff_adjoint_dev(work.camera, work.width, work.height, work.spp, first_frame,
+            ctx.diff_adjoint_dev(work.camera, work.width, work.height, work.spp, grad_frame,
                                  work.depth, p, pmap, gi.data_ptr(), g.data_ptr(), s)
-            return g.to(params.dtype), None, None, None, None
+            return g.to(params.dtype), None, None, None, None, None
 
     return DiffRender
 
@@ -77,39 +77,59 @@ def _autograd_function():
 _FN = None
 
 
-def render(params, ctx, work, pmap, first_frame: int = 0):
-    """Differentiable render of `work` with parameter tensor `params` (8, 4)."""
+def render(params, ctx, work, pmap, first_frame: int = 0, grad_frame: int = None):
+    """Differentiable render of `work` with parameter tensor `params` (8, 4).
+
+    The image uses frames first_frame..first_frame+spp-1.  The backward pass
+    replays frames from `grad_frame` (default: the same frames, giving the
+    exact derivative of this image).  A different range decorrelates the
+    image from its derivative: the gradient of a nonlinear loss such as
+    mean((I - T)^2) is then unbiased, 2 (E[I] - T) E[dI]."""
     global _FN
     if _FN is None:
         _FN = _autograd_function()
-    return _FN.apply(params, ctx, work, pmap, first_frame)
+    return _FN.apply(params, ctx, work, pmap, first_frame, grad_frame)
 
 
 def optimize(ctx, inv: InverseWorkload, steps: int = 200, lr: float = 0.02, target_frame: int = 100_000,
-             log_every: int = 0):
-    """Adam on mean((I - I*)^2) over RGB, the target rendered with the hidden
-    params from an independent PCG range.  Returns (params, history)."""
+             log_every: int = 0, decorrelate: bool = True, target_spp: int = None):
+    """Adam on mean((I - I*)^2) over RGB, the target I* rendered with the
+    hidden params from PCG frames at `target_frame` with `target_spp` samples
+    (default: the workload's spp).  Adam converges to E[I] = I*, so the
+    target's own noise sets the floor of the albedo error.  With
+    `decorrelate`, step k renders frames 2k*spp.. and replays frames
+    (2k+1)*spp.. for the gradient (fresh, independent samples each step);
+    otherwise every step renders and replays frames 0..spp-1.  Returns
+    (params, history of (loss, max albedo error, mean albedo error))."""
+    import dataclasses
+
     import torch
 
     w = inv.work
     dev = torch.device("cuda:0")
     hidden = torch.tensor(inv.hidden, device=dev)
+    tw = dataclasses.replace(w, spp=target_spp or w.spp)
     with torch.no_grad():
-        target = render(hidden, ctx, w, inv.pmap, target_frame)[..., :3].clone()
+        target = render(hidden, ctx, tw, inv.pmap, target_frame)[..., :3].clone()
     params = torch.tensor(inv.init, device=dev, requires_grad=True)
     opt = torch.optim.Adam([params], lr=lr)
     hist = []
     for it in range(steps):
         opt.zero_grad()
-        img = render(params, ctx, w, inv.pmap, 0)
+        if decorrelate:
+            img = render(params, ctx, w, inv.pmap, 2 * it * w.spp, (2 * it + 1) * w.spp)
+        else:
+            img = render(params, ctx, w, inv.pmap, 0)
         loss = torch.mean((img[..., :3] - target) ** 2)
         loss.backward()
         opt.step()
         with torch.no_grad():
             params[:, :3].clamp_(0.01, 1.0)
             params[:, 3].clamp_(0.0, 10.0)
-        err = float(torch.max(torch.abs(params[:, :3] - hidden[:, :3])).item())
-        hist.append((float(loss.item()), err))
+        dev_abs = torch.abs(params[:, :3] - hidden[:, :3])
+        err, mean_err = float(torch.max(dev_abs).item()), float(torch.mean(dev_abs).item())
+        hist.append((float(loss.item()), err, mean_err))
         if log_every and (it % log_every == 0 or it == steps - 1):
-            print(f"adam {it}: loss {hist[-1][0]:.4e} max|albedo - hidden| {err:.4f}", flush=True)
+            print(f"adam {it}: loss {hist[-1][0]:.4e} |albedo - hidden| max {err:.4f} mean {mean_err:.4f}",
+                  flush=True)
     return params.detach().cpu().numpy(), hist

@@ -143,7 +143,22 @@ def test_autograd_binding_and_adam(ctx, mid):
     np.testing.assert_allclose(p.grad.cpu().numpy(), g, rtol=1e-6, atol=0)
     # Target from the same PCG frames: the loss is exactly zero at the hidden
     # parameters, so Adam must drive the albedo error down.
-    params, hist = diff.optimize(ctx, inv, steps=150, lr=0.03, target_frame=0)
+    params, hist = diff.optimize(ctx, inv, steps=150, lr=0.03, target_frame=0, decorrelate=False)
     print(f"adam: loss {hist[0][0]:.3e} -> {hist[-1][0]:.3e}, max albedo err {hist[0][1]:.3f} -> {hist[-1][1]:.3f}")
     assert hist[-1][0] < 0.05 * hist[0][0]
     assert hist[-1][1] < 0.25 * hist[0][1]
+
+
+@pytest.mark.gpu
+def test_inverse_rendering_recovers_hidden_albedos(ctx):
+    # The full inverse-rendering workload: 512 x 512, 32 spp, depth 6, 8
+    # materials with hidden albedos U[0.1, 0.9]; target from independent PCG
+    # frames at 8x the spp; 200 Adam steps with decorrelated primal and replay
+    # frames.  Deterministic (fixed frame ranges), so the bounds are fixed-input
+    # checks with margin over the measured values.
+    inv = diff.inverse_workload()
+    ctx.upload_scene(inv.work.buffers)
+    params, hist = diff.optimize(ctx, inv, steps=200, target_spp=8 * inv.work.spp)
+    print(f"albedo err max {hist[0][1]:.3f} -> {hist[-1][1]:.3f}, mean {hist[0][2]:.4f} -> {hist[-1][2]:.4f}")
+    assert hist[-1][2] < 0.15 * hist[0][2]
+    assert hist[-1][1] < 0.1
