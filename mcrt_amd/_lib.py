@@ -15,7 +15,8 @@ import numpy as np
 from . import types as T
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libmcrt_hip.so")
+# MCRT_HIP_LIB selects another build of the library (e.g. to compare two).
+LIB_PATH = os.environ.get("MCRT_HIP_LIB") or os.path.join(_HERE, "libmcrt_hip.so")
 
 MCRT_OK = 0
 _ERR_NAMES = {

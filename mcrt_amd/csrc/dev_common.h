@@ -60,7 +60,8 @@ struct DevScene {
     const GpuMaterial *mats;
     const float4 *env;
     uint32_t env_w, env_h;
-    uint32_t stack_depth;  // traversal stack entries needed per lane
+    uint32_t stack_depth;     // traversal stack entries needed per lane
+    uint32_t postpone_leaves; // host-side choice of the traversal loop variant
 };
 
 // ---------------------------------------------------------------------------
@@ -364,8 +365,9 @@ __device__ __forceinline__ Best trace_one(const DevScene &sc, V3 o, V3 d, float 
 
 // IO policy: count(), work(), load(slot, o, d, t_min, t_max), store(slot, best).
 // kSpheres = false compiles the variant for scenes without spheres (no
-// sphere phase state), launched when the sphere BVH is empty.
-template <int kBlock, bool kSpheres, class IO>
+// sphere phase state), launched when the sphere BVH is empty.  kPostpone
+// selects the postponed-leaf node loop (see below).
+template <int kBlock, bool kSpheres, bool kPostpone, class IO>
 __device__ __forceinline__ void trace_persistent(const DevScene &sc, const IO &io, uint32_t *lds,
                                                  uint32_t *ovf, uint32_t ovf_stride) {
     const uint32_t lane = __lane_id();
@@ -378,6 +380,7 @@ __device__ __forceinline__ void trace_persistent(const DevScene &sc, const IO &i
     V3 o = v3(0, 0, 0), d = v3(0, 0, 0), inv = v3(0, 0, 0);
     float t_min = 0.0f, t_q = 0.0f, t_cur = 0.0f;  // query and current-phase upper bounds
     uint32_t cur = kNone, sp = 0;
+    uint32_t leaf = kNone;  // postponed leaf
     bool sphere_phase = false;
     Best best, sbest;
     best.t = 0.0f;
@@ -415,6 +418,7 @@ __device__ __forceinline__ void trace_persistent(const DevScene &sc, const IO &i
                 t_cur = t_q;
                 cur = enter_root(sc, sphere_phase, o, inv, t_min, t_cur);
                 sp = 0;
+                leaf = kNone;
             }
             wbase += min(avail, nidle);
         }
@@ -423,16 +427,33 @@ __device__ __forceinline__ void trace_persistent(const DevScene &sc, const IO &i
             continue;
         }
         // ---- internal nodes until every lane holds a leaf or is done -------
+        // kPostpone: a lane that reaches a leaf postpones it and keeps
+        // descending while other lanes still search for theirs; its extra
+        // steps cull with an older (larger) best t, so they only visit more
+        // nodes.  This raises lane utilisation where the scene is cache
+        // resident and costs extra fetches where it is not.
         for (;;) {
+            if (kPostpone && slot != kNone && leaf == kNone && cur != kNone && (cur & kLeafBit)) {
+                leaf = cur;
+                cur = sp ? st.get(--sp) : kNone;
+            }
             const bool in = slot != kNone && cur != kNone && !(cur & kLeafBit);
-            if (__ballot(in) == 0) break;
+            if (__ballot(kPostpone ? (in && leaf == kNone) : in) == 0) break;
             if (in) {
                 const float4 *nodes = (kSpheres && sphere_phase) ? sc.snodes : sc.mnodes;
                 node_step(nodes, o, inv, t_min, t_cur, best.t, cur, sp, st);
             }
         }
         // ---- leaves ----------------------------------------------------------
-        if (slot != kNone && cur != kNone) {
+        if (kPostpone) {
+            if (slot != kNone && leaf != kNone) {
+                if (kSpheres && sphere_phase)
+                    sphere_leaf(sc, leaf & ~kLeafBit, o, d, t_min, t_cur, best);
+                else
+                    tri_leaf(sc, leaf & ~kLeafBit, o, d, t_min, t_cur, best);
+                leaf = kNone;
+            }
+        } else if (slot != kNone && cur != kNone) {
             if (kSpheres && sphere_phase)
                 sphere_leaf(sc, cur & ~kLeafBit, o, d, t_min, t_cur, best);
             else
@@ -440,7 +461,7 @@ __device__ __forceinline__ void trace_persistent(const DevScene &sc, const IO &i
             cur = sp ? st.get(--sp) : kNone;
         }
         // ---- phase end: spheres -> mesh (world_hit, :471-479) or finish ----
-        if (slot != kNone && cur == kNone) {
+        if (slot != kNone && cur == kNone && leaf == kNone) {
             if (kSpheres && sphere_phase) {
                 sbest = best;
                 sphere_phase = false;

@@ -150,8 +150,8 @@ hipError_t launch_shade(const ShadeArgs &a, uint32_t grid, hipStream_t s);
 hipError_t launch_accumulate(const AccArgs &a, hipStream_t s);
 hipError_t launch_trace(const TraceArgs &a, uint32_t grid, hipStream_t s);
 hipError_t launch_bilateral(const BilateralArgs &a, hipStream_t s);
-// which: 0 extend, 1 shade, 2 trace
-hipError_t occupancy(int which, bool spheres, int *blocks_per_cu);
+// which: 0 extend, 1 shade, 2 trace (the traversal variant the scene selects)
+hipError_t occupancy(int which, const DevScene &sc, int *blocks_per_cu);
 int trace_block();
 uint32_t short_stack();
 

@@ -62,10 +62,10 @@ struct ExtIO {
     }
 };
 
-template <int kBlock, bool kSpheres>
+template <int kBlock, bool kSpheres, bool kPostpone>
 __global__ __launch_bounds__(kBlock) void k_extend(ExtArgs a) {
     __shared__ uint32_t lds[kShortStack * kBlock];
-    trace_persistent<kBlock, kSpheres>(a.scene, ExtIO{a}, lds, a.ovf, a.ovf_stride);
+    trace_persistent<kBlock, kSpheres, kPostpone>(a.scene, ExtIO{a}, lds, a.ovf, a.ovf_stride);
 }
 
 // ---------------------------------------------------------------------------
@@ -189,10 +189,10 @@ struct TraceIO {
     }
 };
 
-template <int kBlock, bool kSpheres>
+template <int kBlock, bool kSpheres, bool kPostpone>
 __global__ __launch_bounds__(kBlock) void k_trace(TraceArgs a) {
     __shared__ uint32_t lds[kShortStack * kBlock];
-    trace_persistent<kBlock, kSpheres>(a.scene, TraceIO{a}, lds, a.ovf, a.ovf_stride);
+    trace_persistent<kBlock, kSpheres, kPostpone>(a.scene, TraceIO{a}, lds, a.ovf, a.ovf_stride);
 }
 
 __global__ void k_fill_u32(uint32_t *p, uint32_t n, uint32_t v) {
@@ -215,12 +215,33 @@ hipError_t launch_generate(const GenArgs &a, hipStream_t s) {
     return hipGetLastError();
 }
 
+// Traversal kernel variant for a scene: with or without the sphere phase,
+// with or without postponed leaves.
+template <int kBlock, bool kSpheres, bool kPostpone>
+struct ExtendK {
+    static const void *fn() { return reinterpret_cast<const void *>(&k_extend<kBlock, kSpheres, kPostpone>); }
+};
+template <int kBlock, bool kSpheres, bool kPostpone>
+struct TraceK {
+    static const void *fn() { return reinterpret_cast<const void *>(&k_trace<kBlock, kSpheres, kPostpone>); }
+};
+
+template <template <int, bool, bool> class K>
+const void *traversal_kernel(const DevScene &sc) {
+    const bool sph = sc.sroot != kNone, pp = sc.postpone_leaves != 0;
+    if (sph) return pp ? K<kTraceBlock, true, true>::fn() : K<kTraceBlock, true, false>::fn();
+    return pp ? K<kTraceBlock, false, true>::fn() : K<kTraceBlock, false, false>::fn();
+}
+
+template <class Args>
+hipError_t launch_traversal(const void *fn, const Args &a, uint32_t grid, hipStream_t s) {
+    Args copy = a;
+    void *args[] = {&copy};
+    return hipLaunchKernel(fn, dim3(grid), dim3(kTraceBlock), args, 0, s);
+}
+
 hipError_t launch_extend(const ExtArgs &a, uint32_t grid, hipStream_t s) {
-    if (a.scene.sroot != kNone)
-        hipLaunchKernelGGL((k_extend<kTraceBlock, true>), dim3(grid), dim3(kTraceBlock), 0, s, a);
-    else
-        hipLaunchKernelGGL((k_extend<kTraceBlock, false>), dim3(grid), dim3(kTraceBlock), 0, s, a);
-    return hipGetLastError();
+    return launch_traversal(traversal_kernel<ExtendK>(a.scene), a, grid, s);
 }
 
 // `grid` = blocks covering the largest queue of the batch.
@@ -235,23 +256,17 @@ hipError_t launch_accumulate(const AccArgs &a, hipStream_t s) {
 }
 
 hipError_t launch_trace(const TraceArgs &a, uint32_t grid, hipStream_t s) {
-    if (a.scene.sroot != kNone)
-        hipLaunchKernelGGL((k_trace<kTraceBlock, true>), dim3(grid), dim3(kTraceBlock), 0, s, a);
-    else
-        hipLaunchKernelGGL((k_trace<kTraceBlock, false>), dim3(grid), dim3(kTraceBlock), 0, s, a);
-    return hipGetLastError();
+    return launch_traversal(traversal_kernel<TraceK>(a.scene), a, grid, s);
 }
 
-hipError_t occupancy(int which, bool spheres, int *blocks_per_cu) {
+hipError_t occupancy(int which, const DevScene &sc, int *blocks_per_cu) {
     const void *f = nullptr;
     if (which == 0)
-        f = spheres ? reinterpret_cast<const void *>(&k_extend<kTraceBlock, true>)
-                    : reinterpret_cast<const void *>(&k_extend<kTraceBlock, false>);
+        f = traversal_kernel<ExtendK>(sc);
     else if (which == 1)
         f = reinterpret_cast<const void *>(&k_shade);
     else
-        f = spheres ? reinterpret_cast<const void *>(&k_trace<kTraceBlock, true>)
-                    : reinterpret_cast<const void *>(&k_trace<kTraceBlock, false>);
+        f = traversal_kernel<TraceK>(sc);
     return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, f, which == 1 ? 256 : kTraceBlock, 0);
 }
 
