@@ -16,11 +16,14 @@
 // each of the 9 taps multiplies A = (16 pixels of a row) x (32 input
 // channels) by B = (32 input channels) x (16 output channels), twice for the
 // 32 outputs.  The input tile with its 1-pixel halo (zero padded at the
-// image border) and the 9 x 32 x 32 weights are staged in LDS once.
+// image border) is staged in LDS; the 9 x 32 x 32 weights live in registers
+// as MFMA B fragments for the whole (grid-stride) block.
 // The same kernel computes the data gradient (flipped, transposed weights,
 // ReLU mask of the layer output fused into the tile load).
 // The weight gradient is a second MFMA GEMM with the pixels as the K axis.
 #include <hip/hip_runtime.h>
+
+#include <algorithm>
 
 #include "kernels.h"
 
@@ -81,97 +84,123 @@ __global__ __launch_bounds__(256) void k_pack(PackArgs a) {
 // ---------------------------------------------------------------------------
 // 3x3 convolution, implicit GEMM on MFMA
 // ---------------------------------------------------------------------------
+// A block loops over tiles (grid-stride) and keeps the layer's B fragments
+// in registers for its lifetime: lane (m, kq) holds
+// w[tap][16 h + m][8 kq .. 8 kq + 7] for the 9 taps and both output halves
+// h, loaded from global memory once.  Per tile, the 18 x 18 input tile is
+// staged in LDS; wave w owns output rows 4w .. 4w+3, which read input rows
+// 4w .. 4w+5 of the halo tile.  Each A fragment (16 pixels of one input row
+// at one horizontal tap kx, 32 channels) is read from LDS once and feeds
+// every output row r it contributes to (vertical tap ky = input row - r).
+// kFinal: the last layer, whose 3 outputs need only the first half.
+constexpr int kConvGrid = 1024;  // 256 CUs x 4 resident blocks
+
+template <bool kFinal>
 __global__ __launch_bounds__(256) void k_conv3x3(ConvArgs a) {
     __shared__ bf16 s_in[kH * kH * kPS];
-    __shared__ bf16 s_w[9 * kC * kPS];
     const int tid = threadIdx.x;
-    const int x0 = blockIdx.x * kT, y0 = blockIdx.y * kT, b = blockIdx.z;
-    // Weights [tap][co][ci] -> LDS rows of kPS.
-    for (int i = tid; i < 9 * kC * 4; i += 256) {
-        const int row = i >> 2, q = i & 3;
-        *reinterpret_cast<uint4 *>(&s_w[row * kPS + q * 8]) =
-            *reinterpret_cast<const uint4 *>(a.w + row * kC + q * 8);
-    }
-    // Input tile with halo; zero outside the image.
-    for (int i = tid; i < kH * kH * 4; i += 256) {
-        const int pix = i >> 2, q = i & 3;
-        const int hy = pix / kH, hx = pix - hy * kH;
-        const int gy = y0 + hy - 1, gx = x0 + hx - 1;
-        uint4 v = make_uint4(0, 0, 0, 0);
-        if (gy >= 0 && gy < a.H && gx >= 0 && gx < a.W) {
-            const size_t off = (((size_t)b * a.H + gy) * a.W + gx) * kC + q * 8;
-            v = *reinterpret_cast<const uint4 *>(a.in + off);
-            if (a.mask) v = apply_mask(v, *reinterpret_cast<const uint4 *>(a.mask + off));
-        }
-        *reinterpret_cast<uint4 *>(&s_in[pix * kPS + q * 8]) = v;
-    }
-    __syncthreads();
     const int wave = tid >> 6, lane = tid & 63;
     const int m = lane & 15, kq = lane >> 4;
-    floatx4 acc[4][2];
+    constexpr int kHalves = kFinal ? 1 : 2;
+    bf16x8 wfrag[9][kHalves];
 #pragma unroll
-    for (int r = 0; r < 4; ++r) acc[r][0] = acc[r][1] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+    for (int tap = 0; tap < 9; ++tap)
 #pragma unroll
-    for (int tap = 0; tap < 9; ++tap) {
-        const int ky = tap / 3, kx = tap - ky * 3;
-        const bf16x8 b0 = *reinterpret_cast<const bf16x8 *>(&s_w[(tap * kC + m) * kPS + kq * 8]);
-        const bf16x8 b1 = *reinterpret_cast<const bf16x8 *>(&s_w[(tap * kC + 16 + m) * kPS + kq * 8]);
+        for (int h = 0; h < kHalves; ++h)
+            wfrag[tap][h] = *reinterpret_cast<const bf16x8 *>(a.w + (tap * kC + 16 * h + m) * kC + kq * 8);
+    float bias[kHalves];
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const int y = wave * 4 + r;
-            const bf16x8 av =
-                *reinterpret_cast<const bf16x8 *>(&s_in[((y + ky) * kH + (m + kx)) * kPS + kq * 8]);
-            acc[r][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, b0, acc[r][0], 0, 0, 0);
-            acc[r][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, b1, acc[r][1], 0, 0, 0);
+    for (int h = 0; h < kHalves; ++h) bias[h] = a.bias ? a.bias[16 * h + m] : 0.0f;
+
+    const int tiles_x = (a.W + kT - 1) / kT, tiles_y = (a.H + kT - 1) / kT;
+    const int per_image = tiles_x * tiles_y, n_tiles = per_image * a.B;
+    for (int tile = blockIdx.x; tile < n_tiles; tile += gridDim.x) {
+        const int b = tile / per_image, rem = tile - b * per_image;
+        const int y0 = (rem / tiles_x) * kT, x0 = (rem % tiles_x) * kT;
+        __syncthreads();  // the previous tile is done with s_in
+        // Input tile with halo; zero outside the image.
+        for (int i = tid; i < kH * kH * 4; i += 256) {
+            const int pix = i >> 2, q = i & 3;
+            const int hy = pix / kH, hx = pix - hy * kH;
+            const int gy = y0 + hy - 1, gx = x0 + hx - 1;
+            uint4 v = make_uint4(0, 0, 0, 0);
+            if (gy >= 0 && gy < a.H && gx >= 0 && gx < a.W) {
+                const size_t off = (((size_t)b * a.H + gy) * a.W + gx) * kC + q * 8;
+                v = *reinterpret_cast<const uint4 *>(a.in + off);
+                if (a.mask) v = apply_mask(v, *reinterpret_cast<const uint4 *>(a.mask + off));
+            }
+            *reinterpret_cast<uint4 *>(&s_in[pix * kPS + q * 8]) = v;
         }
-    }
-    // D layout: lane holds rows 4*kq + j (pixel x), column m (output channel).
-    if (a.mode == 2) {
-        // Final layer: out_log = log1p(max(noisy, 0)) + conv + bias, channels 0..2.
+        __syncthreads();
+        floatx4 acc[4][kHalves];
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const int gy = y0 + wave * 4 + r;
+        for (int r = 0; r < 4; ++r)
 #pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                const int gx = x0 + 4 * kq + j;
-                if (gy < a.H && gx < a.W && m < 4) {
-                    const size_t p = ((size_t)b * a.H + gy) * a.W + gx;
-                    float *o = reinterpret_cast<float *>(a.out_log + p);
-                    if (m < 3) {
-                        const float nz = reinterpret_cast<const float *>(a.noisy + p)[m];
-                        o[m] = log1pf(fmaxf(nz, 0.0f)) + (acc[r][0][j] + (a.bias ? a.bias[m] : 0.0f));
-                    } else {
-                        o[3] = 0.0f;
-                    }
+            for (int h = 0; h < kHalves; ++h) acc[r][h] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int ir = 0; ir < 6; ++ir) {
+#pragma unroll
+            for (int kx = 0; kx < 3; ++kx) {
+                const bf16x8 av =
+                    *reinterpret_cast<const bf16x8 *>(&s_in[((wave * 4 + ir) * kH + (m + kx)) * kPS + kq * 8]);
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int ky = ir - r;
+                    if (ky < 0 || ky > 2) continue;
+#pragma unroll
+                    for (int h = 0; h < kHalves; ++h)
+                        acc[r][h] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, wfrag[ky * 3 + kx][h], acc[r][h],
+                                                                             0, 0, 0);
                 }
             }
         }
-        return;
-    }
-    __syncthreads();  // s_in is reused as the output tile
+        // D layout: lane holds rows 4*kq + j (pixel x), column m (output channel).
+        if (kFinal) {
+            // out_log = log1p(max(noisy, 0)) + conv + bias, channels 0..2.
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-        const int y = wave * 4 + r;
+            for (int r = 0; r < 4; ++r) {
+                const int gy = y0 + wave * 4 + r;
 #pragma unroll
-        for (int n = 0; n < 2; ++n) {
-            const int co = 16 * n + m;
-            const float bias = a.bias ? a.bias[co] : 0.0f;
+                for (int j = 0; j < 4; ++j) {
+                    const int gx = x0 + 4 * kq + j;
+                    if (gy < a.H && gx < a.W && m < 4) {
+                        const size_t p = ((size_t)b * a.H + gy) * a.W + gx;
+                        float *o = reinterpret_cast<float *>(a.out_log + p);
+                        if (m < 3) {
+                            const float nz = reinterpret_cast<const float *>(a.noisy + p)[m];
+                            o[m] = log1pf(fmaxf(nz, 0.0f)) + (acc[r][0][j] + bias[0]);
+                        } else {
+                            o[3] = 0.0f;
+                        }
+                    }
+                }
+            }
+            continue;
+        }
+        __syncthreads();  // s_in is reused as the output tile
 #pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                float v = acc[r][n][j] + bias;
-                if (a.mode == 0) v = fmaxf(v, 0.0f);
-                s_in[(y * kT + 4 * kq + j) * kPS + co] = f2b(v);
+        for (int r = 0; r < 4; ++r) {
+            const int y = wave * 4 + r;
+#pragma unroll
+            for (int h = 0; h < kHalves; ++h) {
+                const int co = 16 * h + m;
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    float v = acc[r][h][j] + bias[h];
+                    if (a.mode == 0) v = fmaxf(v, 0.0f);
+                    s_in[(y * kT + 4 * kq + j) * kPS + co] = f2b(v);
+                }
             }
         }
-    }
-    __syncthreads();
-    for (int i = tid; i < kT * kT * 4; i += 256) {
-        const int pix = i >> 2, q = i & 3;
-        const int y = pix >> 4, x = pix & 15;
-        const int gy = y0 + y, gx = x0 + x;
-        if (gy < a.H && gx < a.W)
-            *reinterpret_cast<uint4 *>(a.out + (((size_t)b * a.H + gy) * a.W + gx) * kC + q * 8) =
-                *reinterpret_cast<const uint4 *>(&s_in[pix * kPS + q * 8]);
+        __syncthreads();
+        for (int i = tid; i < kT * kT * 4; i += 256) {
+            const int pix = i >> 2, q = i & 3;
+            const int y = pix >> 4, x = pix & 15;
+            const int gy = y0 + y, gx = x0 + x;
+            if (gy < a.H && gx < a.W)
+                *reinterpret_cast<uint4 *>(a.out + (((size_t)b * a.H + gy) * a.W + gx) * kC + q * 8) =
+                    *reinterpret_cast<const uint4 *>(&s_in[pix * kPS + q * 8]);
+        }
     }
 }
 
@@ -394,8 +423,12 @@ hipError_t launch_pack(const PackArgs &a, hipStream_t s) {
 }
 
 hipError_t launch_conv3x3(const ConvArgs &a, hipStream_t s) {
-    dim3 grid((a.W + kT - 1) / kT, (a.H + kT - 1) / kT, a.B);
-    hipLaunchKernelGGL(k_conv3x3, grid, dim3(256), 0, s, a);
+    const size_t tiles = (size_t)((a.W + kT - 1) / kT) * ((a.H + kT - 1) / kT) * a.B;
+    const dim3 grid((uint32_t)std::min<size_t>(tiles, kConvGrid));
+    if (a.mode == 2)
+        hipLaunchKernelGGL(k_conv3x3<true>, grid, dim3(256), 0, s, a);
+    else
+        hipLaunchKernelGGL(k_conv3x3<false>, grid, dim3(256), 0, s, a);
     return hipGetLastError();
 }
 
