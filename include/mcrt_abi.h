@@ -200,6 +200,17 @@ int mcrt_render_dev(mcrt_ctx *ctx, const CameraUniform *camera,
                     const float *accum_in_dev, float *out_accum_dev,
                     float *out_gbuf_dev, void *stream);
 
+/* Opt-in next-event estimation for the mcrt_render* calls of this context
+ * (default off: the reference estimator, BSDF sampling only,
+ * path_trace.wgsl:583-620).  When on, every Lambertian vertex also samples
+ * one emitter (emissive triangles and the emissive spheres that binding 12
+ * lists, src/gpu_layout.rs:51-52, src/scene.rs:912-923) and adds its light
+ * through a shadow ray, combined with BSDF sampling by multiple importance
+ * sampling (power heuristic).  The expectation is the reference
+ * estimator's; the noise is lower.  BSDF draws are unchanged: light samples
+ * use a separate PCG stream. */
+int mcrt_set_nee(mcrt_ctx *ctx, int enabled);
+
 /* mcrt_render_dev plus a first-hit albedo buffer (W*H RGBA32F, last frame;
  * dielectric = 1, sky = 0).  No counterpart in the reference, whose G-buffer
  * holds normal and depth only: it is the extra input of the learned

@@ -77,6 +77,7 @@ def load() -> C.CDLL:
         "mcrt_build_mesh_bvh": (C.c_int, [_vp, _u32, _vp, _u32, _vp, C.POINTER(_u32), _vp, _vp]),
         "mcrt_build_sphere_bvh": (C.c_int, [_vp, _u32, _vp, C.POINTER(_u32), _vp, _vp]),
         "mcrt_set_profiling": (C.c_int, [_vp, C.c_int]),
+        "mcrt_set_nee": (C.c_int, [_vp, C.c_int]),
         "mcrt_kernel_stats": (C.c_int, [_vp, C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_uint64)]),
         "mcrt_reset_stats": (C.c_int, [_vp]),
         "mcrt_render_counters": (C.c_int, [_vp, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
@@ -114,7 +115,7 @@ def exported_symbols() -> list:
         "mcrt_create", "mcrt_destroy", "mcrt_last_error", "mcrt_upload_scene", "mcrt_render",
         "mcrt_render_dev", "mcrt_denoise_bilateral", "mcrt_denoise_bilateral_dev",
         "mcrt_trace_closest", "mcrt_trace_closest_dev", "mcrt_build_mesh_bvh",
-        "mcrt_build_sphere_bvh", "mcrt_set_profiling", "mcrt_kernel_stats", "mcrt_reset_stats",
+        "mcrt_build_sphere_bvh", "mcrt_set_profiling", "mcrt_set_nee", "mcrt_kernel_stats", "mcrt_reset_stats",
         "mcrt_render_counters", "mcrt_mesh_voxel_chunks", "mcrt_diff_render", "mcrt_diff_render_dev",
         "mcrt_diff_adjoint", "mcrt_diff_adjoint_dev", "mcrt_render_aux_dev", "mcrt_denoiser_create",
         "mcrt_denoiser_destroy", "mcrt_denoiser_param_count", "mcrt_denoiser_get_weights",
@@ -338,6 +339,10 @@ class Context:
 
     def set_profiling(self, enabled: bool) -> None:
         self._check(self._lib.mcrt_set_profiling(self._h, 1 if enabled else 0))
+
+    def set_nee(self, enabled: bool) -> None:
+        """Next-event estimation with MIS for later renders (default off)."""
+        self._check(self._lib.mcrt_set_nee(self._h, 1 if enabled else 0))
 
     def kernel_stats(self, name: str):
         ms = C.c_double(0.0)

@@ -52,12 +52,15 @@ struct mcrt_ctx {
     // scene
     bool has_scene = false;
     DevBuf mnodes, mtris, tshade, snodes, sph, mats, env;
+    DevBuf emit, emit_cdf, tri_emit, sph_emit;  // next-event estimation emitters
     DevBuf ovf;  // traversal-stack overflow for trees deeper than the LDS short stack
     DevScene scene{};
     uint32_t n_tris = 0, n_spheres = 0;
+    bool nee = false;  // next-event estimation (mcrt_set_nee)
     // work buffers
     DevBuf A[2], B[2], C[2], H, samples;
-    size_t cap_paths = 0;
+    DevBuf P[2], S_o, S_d, S_c;  // next-event estimation path pdf and shadow queue
+    size_t cap_paths = 0, cap_nee = 0;
     DevBuf counters;
     size_t cap_counters = 0;
     DevBuf host_accum, host_gbuf, host_out, host_accum_in;
@@ -333,7 +336,9 @@ extern "C" void mcrt_destroy(mcrt_ctx *c) {
                       &c->env, &c->A[0], &c->A[1], &c->B[0], &c->B[1], &c->C[0], &c->C[1], &c->H,
                       &c->samples, &c->counters, &c->host_accum, &c->host_gbuf, &c->host_out,
                       &c->host_accum_in, &c->trace_o, &c->trace_d, &c->trace_id, &c->trace_t,
-                      &c->trace_work, &c->spatial, &c->diff_partial, &c->diff_grad})
+                      &c->trace_work, &c->spatial, &c->diff_partial, &c->diff_grad, &c->emit,
+                      &c->emit_cdf, &c->tri_emit, &c->sph_emit, &c->P[0], &c->P[1], &c->S_o, &c->S_d,
+                      &c->S_c})
         release(*b);
     for (Launch &l : c->pending) {
         (void)hipEventDestroy(l.start);
@@ -423,6 +428,67 @@ extern "C" int mcrt_upload_scene(mcrt_ctx *c, const GpuSphere *spheres, uint32_t
     std::vector<float4> denv((size_t)env_w * env_h);
     std::memcpy(denv.data(), env_rgba32f, denv.size() * sizeof(float4));
 
+    // Emitters for next-event estimation: triangles and spheres whose material
+    // is emissive (emission = colour x strength, :608-610), chosen with
+    // probability proportional to area x luminance.  Triangle area from the
+    // same FP32 edges the kernels use.
+    std::vector<float4> demit;
+    std::vector<float> dcdf;
+    std::vector<uint32_t> dtri_emit, dsph_emit;
+    {
+        struct Em {
+            uint32_t ref;
+            float area, le[3], w;
+        };
+        std::vector<Em> em;
+        auto emission = [&](uint32_t mat, float le[3]) {
+            const GpuMaterial &g = materials->materials[std::min<uint32_t>(mat, MCRT_MAX_MATERIALS - 1)];
+            if (g.type_pad[0] != MCRT_MAT_EMISSIVE) return false;
+            for (int k = 0; k < 3; ++k) le[k] = g.ior_pad[0] * g.albedo_fuzz[k];
+            return true;
+        };
+        auto lum = [](const float le[3]) { return 0.2126f * le[0] + 0.7152f * le[1] + 0.0722f * le[2]; };
+        for (uint32_t i = 0; i < n_tris; ++i) {
+            Em e{};
+            if (!emission(tris[i].mat_idx & 0x7FFFFFFFu, e.le)) continue;
+            const float4 e1 = dt[3 * (size_t)i + 1], e2 = dt[3 * (size_t)i + 2];
+            const float cx = e1.y * e2.z - e1.z * e2.y, cy = e1.z * e2.x - e1.x * e2.z,
+                        cz = e1.x * e2.y - e1.y * e2.x;
+            e.area = 0.5f * std::sqrt(cx * cx + cy * cy + cz * cz);
+            e.w = e.area * lum(e.le);
+            e.ref = i;
+            if (e.w > 0.0f && std::isfinite(e.w)) em.push_back(e);
+        }
+        for (uint32_t i = 0; i < n_spheres; ++i) {
+            Em e{};
+            if (!emission(spheres[i].mat_and_pad[0], e.le)) continue;
+            const float r = spheres[i].center_r[3];
+            e.area = 4.0f * 3.14159265358979f * r * r;
+            e.w = e.area * lum(e.le);
+            e.ref = 0x80000000u | i;
+            if (e.w > 0.0f && std::isfinite(e.w)) em.push_back(e);
+        }
+        double total = 0.0;
+        for (const Em &e : em) total += e.w;
+        if (!em.empty()) {
+            dtri_emit.assign(std::max<uint32_t>(n_tris, 1), kNone);
+            dsph_emit.assign(std::max<uint32_t>(n_spheres, 1), kNone);
+            double run = 0.0;
+            for (size_t k = 0; k < em.size(); ++k) {
+                const Em &e = em[k];
+                const float prob = (float)(e.w / total);
+                run += e.w;
+                dcdf.push_back(k + 1 == em.size() ? 1.0f : (float)(run / total));
+                demit.push_back(make_float4(prob, ubits(e.ref), e.area, 0.0f));
+                demit.push_back(make_float4(e.le[0], e.le[1], e.le[2], 0.0f));
+                if (e.ref & 0x80000000u)
+                    dsph_emit[e.ref & 0x7FFFFFFFu] = (uint32_t)k;
+                else
+                    dtri_emit[e.ref] = (uint32_t)k;
+            }
+        }
+    }
+
     if ((rc = upload(c, c->mnodes, mt.nodes.data(), mt.nodes.size() * sizeof(float4)))) return rc;
     if ((rc = upload(c, c->mtris, dt.data(), dt.size() * sizeof(float4)))) return rc;
     if ((rc = upload(c, c->tshade, dsh.data(), dsh.size() * sizeof(float4)))) return rc;
@@ -430,9 +496,21 @@ extern "C" int mcrt_upload_scene(mcrt_ctx *c, const GpuSphere *spheres, uint32_t
     if ((rc = upload(c, c->sph, ds.data(), ds.size() * sizeof(float4)))) return rc;
     if ((rc = upload(c, c->mats, materials->materials, sizeof(materials->materials)))) return rc;
     if ((rc = upload(c, c->env, denv.data(), denv.size() * sizeof(float4)))) return rc;
+    if (!demit.empty()) {
+        if ((rc = upload(c, c->emit, demit.data(), demit.size() * sizeof(float4))) ||
+            (rc = upload(c, c->emit_cdf, dcdf.data(), dcdf.size() * sizeof(float))) ||
+            (rc = upload(c, c->tri_emit, dtri_emit.data(), dtri_emit.size() * sizeof(uint32_t))) ||
+            (rc = upload(c, c->sph_emit, dsph_emit.data(), dsph_emit.size() * sizeof(uint32_t))))
+            return rc;
+    }
 
     DevScene &d = c->scene;
     d = DevScene{};
+    d.n_emit = (uint32_t)dcdf.size();
+    d.emit = d.n_emit ? (const float4 *)c->emit.p : nullptr;
+    d.emit_cdf = d.n_emit ? (const float *)c->emit_cdf.p : nullptr;
+    d.tri_emit = d.n_emit ? (const uint32_t *)c->tri_emit.p : nullptr;
+    d.sph_emit = d.n_emit ? (const uint32_t *)c->sph_emit.p : nullptr;
     d.mnodes = (const float4 *)c->mnodes.p;
     d.mtris = (const float4 *)c->mtris.p;
     d.tshade = (const float4 *)c->tshade.p;
@@ -472,6 +550,12 @@ extern "C" int mcrt_render_aux_dev(mcrt_ctx *c, const CameraUniform *cam, uint32
                                    float *out_accum_dev, float *out_gbuf_dev,
                                    float *out_albedo_dev, void *stream);
 
+extern "C" int mcrt_set_nee(mcrt_ctx *c, int enabled) {
+    if (!c) return MCRT_ERR_BAD_ARG;
+    c->nee = enabled != 0;
+    return MCRT_OK;
+}
+
 extern "C" int mcrt_render_dev(mcrt_ctx *c, const CameraUniform *cam, uint32_t width,
                                uint32_t height, uint32_t spp, uint32_t first_frame,
                                uint32_t max_depth, const float *accum_in_dev, float *out_accum_dev,
@@ -508,17 +592,30 @@ extern "C" int mcrt_render_aux_dev(mcrt_ctx *c, const CameraUniform *cam, uint32
             if ((rc = ensure(c, *b, cap * sizeof(float4)))) return rc;
         c->cap_paths = cap;
     }
+    const bool nee = c->nee;
+    if (nee && cap > c->cap_nee) {
+        for (DevBuf *b : {&c->P[0], &c->P[1], &c->S_o, &c->S_d, &c->S_c}) release(*b);
+        for (DevBuf *b : {&c->S_o, &c->S_d, &c->S_c})
+            if ((rc = ensure(c, *b, cap * sizeof(float4)))) return rc;
+        for (DevBuf *b : {&c->P[0], &c->P[1]})
+            if ((rc = ensure(c, *b, cap * sizeof(float)))) return rc;
+        c->cap_nee = cap;
+    }
     const uint32_t n_batches = (spp + F - 1) / F;
-    const uint32_t per_batch = 3u * (max_depth + 1u);
+    // Per batch: path counts [max_depth+1], extend and shade work counters,
+    // then shadow-ray counts and occlusion work counters per bounce.
+    const uint32_t per_batch = 3u * (max_depth + 1u) + 2u * max_depth;
     const size_t n_counters = (size_t)per_batch * n_batches;
     if ((rc = ensure(c, c->counters, n_counters * sizeof(uint32_t)))) return rc;
     uint32_t *counters = (uint32_t *)c->counters.p;
     HIP_TRY(c, launch_fill_u32(counters, (uint32_t)n_counters, 0u, s));
 
-    uint32_t g_ext = 0;
+    uint32_t g_ext = 0, g_occ = 0;
     grid_for(c, 0, &g_ext);
+    if (nee) grid_for(c, 3, &g_occ);
+    const uint32_t ovf_threads = std::max(g_ext, g_occ) * (uint32_t)trace_block();
     uint32_t *ovf = nullptr;
-    if ((rc = ensure_overflow(c, g_ext * (uint32_t)trace_block(), &ovf))) return rc;
+    if ((rc = ensure_overflow(c, ovf_threads, &ovf))) return rc;
 
     const uint32_t last_frame = first_frame + spp - 1u;
     for (uint32_t bi = 0; bi < n_batches; ++bi) {
@@ -528,6 +625,8 @@ extern "C" int mcrt_render_aux_dev(mcrt_ctx *c, const CameraUniform *cam, uint32
         uint32_t *cnt = counters + (size_t)bi * per_batch;  // [max_depth+1] path counts
         uint32_t *wext = cnt + (max_depth + 1u);
         uint32_t *wsh = wext + (max_depth + 1u);
+        uint32_t *scnt = wsh + (max_depth + 1u);  // [max_depth] shadow rays per bounce
+        uint32_t *wocc = scnt + max_depth;
         HIP_TRY(c, launch_fill_u32(cnt, 1u, np, s));
         GenArgs g{};
         g.cam = *cam;
@@ -539,6 +638,8 @@ extern "C" int mcrt_render_aux_dev(mcrt_ctx *c, const CameraUniform *cam, uint32
         g.A = (float4 *)c->A[0].p;
         g.B = (float4 *)c->B[0].p;
         g.C = (float4 *)c->C[0].p;
+        g.samples = nee ? (float4 *)c->samples.p : nullptr;
+        g.P = nee ? (float *)c->P[0].p : nullptr;
         {
             Timed t(c, "generate", s);
             HIP_TRY(c, launch_generate(g, s));
@@ -577,9 +678,32 @@ extern "C" int mcrt_render_aux_dev(mcrt_ctx *c, const CameraUniform *cam, uint32
             sh.gbuf_frame_index = last_frame - f0;
             sh.wh = wh;
             sh.max_depth = max_depth;
+            if (nee) {
+                sh.nee = 1u;
+                sh.P = (const float *)c->P[cur].p;
+                sh.P_out = (float *)c->P[nxt].p;
+                sh.S_o = (float4 *)c->S_o.p;
+                sh.S_d = (float4 *)c->S_d.p;
+                sh.S_c = (float4 *)c->S_c.p;
+                sh.shadow_count = scnt + b;
+            }
             {
                 Timed t(c, "shade", s);
                 HIP_TRY(c, launch_shade(sh, (np + 255u) / 256u, s));
+            }
+            if (nee) {
+                OccArgs oa{};
+                oa.scene = c->scene;
+                oa.S_o = (const float4 *)c->S_o.p;
+                oa.S_d = (const float4 *)c->S_d.p;
+                oa.S_c = (const float4 *)c->S_c.p;
+                oa.count = scnt + b;
+                oa.work = wocc + b;
+                oa.samples = (float4 *)c->samples.p;
+                oa.ovf = ovf;
+                oa.ovf_stride = g_occ * (uint32_t)trace_block();
+                Timed t(c, "occlude", s);
+                HIP_TRY(c, launch_occlude(oa, g_occ, s));
             }
         }
         AccArgs acc{};

@@ -62,6 +62,15 @@ struct DevScene {
     uint32_t env_w, env_h;
     uint32_t stack_depth;     // traversal stack entries needed per lane
     uint32_t postpone_leaves; // host-side choice of the traversal loop variant
+    // Emitters for next-event estimation: emissive triangles and spheres,
+    // chosen with probability proportional to area x luminance.
+    //   emit[2e] = (prob, kind | leaf-order index bits, area, -),
+    //   emit[2e+1] = (Le.rgb, -); emit_cdf[e] = sum of prob up to e.
+    // tri_emit / sph_emit map a leaf-order primitive to its emitter or kNone.
+    const float4 *emit;
+    const float *emit_cdf;
+    const uint32_t *tri_emit, *sph_emit;
+    uint32_t n_emit;
 };
 
 // ---------------------------------------------------------------------------

@@ -15,6 +15,8 @@ struct GenArgs {
     uint32_t frame0;   // frame of path 0 (frame_count of the first frame)
     uint32_t n_paths;  // wh * frames in this batch
     float4 *A, *B, *C;
+    float4 *samples;   // next-event estimation: zeroed per path (else null)
+    float *P;          // next-event estimation: BSDF pdf of the arriving ray, 0 = camera
 };
 
 struct ExtArgs {
@@ -40,6 +42,25 @@ struct ShadeArgs {
     uint32_t gbuf_frame_index;  // batch-relative frame whose G-buffer is kept
     uint32_t wh;
     uint32_t max_depth;
+    // Next-event estimation (nee != 0): P / P_out = BSDF solid-angle pdf of
+    // the arriving / leaving ray (0 after the camera or a specular event);
+    // shadow-ray queue S_o = (origin, path id), S_d = (dir, t_max),
+    // S_c = (MIS-weighted contribution, -), appended at *shadow_count.
+    uint32_t nee;
+    const float *P;
+    float *P_out;
+    float4 *S_o, *S_d, *S_c;
+    uint32_t *shadow_count;
+};
+
+struct OccArgs {
+    DevScene scene;
+    const float4 *S_o, *S_d, *S_c;
+    const uint32_t *count;
+    uint32_t *work;
+    float4 *samples;
+    uint32_t *ovf;
+    uint32_t ovf_stride;
 };
 
 struct AccArgs {
@@ -149,8 +170,9 @@ hipError_t launch_extend(const ExtArgs &a, uint32_t grid, hipStream_t s);
 hipError_t launch_shade(const ShadeArgs &a, uint32_t grid, hipStream_t s);
 hipError_t launch_accumulate(const AccArgs &a, hipStream_t s);
 hipError_t launch_trace(const TraceArgs &a, uint32_t grid, hipStream_t s);
+hipError_t launch_occlude(const OccArgs &a, uint32_t grid, hipStream_t s);
 hipError_t launch_bilateral(const BilateralArgs &a, hipStream_t s);
-// which: 0 extend, 1 shade, 2 trace (the traversal variant the scene selects)
+// which: 0 extend, 1 shade, 2 trace, 3 occlude (the traversal variant the scene selects)
 hipError_t occupancy(int which, const DevScene &sc, int *blocks_per_cu);
 int trace_block();
 uint32_t short_stack();

@@ -16,6 +16,7 @@
 // accumulation applies mix(prev, s, 1/(min(f,65535)+1)) frame by frame, so
 // the result does not depend on how paths are scheduled.
 #include "kernels.h"
+#include "nee.h"
 #include "shading.h"
 
 namespace mcrt {
@@ -40,6 +41,8 @@ __global__ __launch_bounds__(256) void k_generate(GenArgs a) {
     a.A[p] = make_float4(ro.x, ro.y, ro.z, __uint_as_float(p));
     a.B[p] = make_float4(rd.x, rd.y, rd.z, __uint_as_float(rng));
     a.C[p] = make_float4(1.0f, 1.0f, 1.0f, __uint_as_float(0u));
+    if (a.samples) a.samples[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (a.P) a.P[p] = 0.0f;
 }
 
 // ---------------------------------------------------------------------------
@@ -75,14 +78,15 @@ __global__ __launch_bounds__(kBlock) void k_extend(ExtArgs a) {
 // the next queue with one atomic per block.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_shade(ShadeArgs a) {
-    __shared__ uint32_t s_wave[4];
-    __shared__ uint32_t s_base;
+    __shared__ uint32_t s_wave[4], s_swave[4];
+    __shared__ uint32_t s_base, s_sbase;
     const uint32_t count = *a.count_in;
     const uint32_t base = blockIdx.x * 256u;
     if (base >= count) return;
     const uint32_t slot = base + threadIdx.x;
-    bool alive = false;
-    float4 nA, nB, nC;
+    bool alive = false, shadow = false;
+    float4 nA, nB, nC, sO, sD, sC;
+    float n_pdf = 0.0f;
     if (slot < count) {
         float4 A = a.A[slot], B = a.B[slot], C = a.C[slot], H = a.H[slot];
         V3 o = xyz(A), d = xyz(B), thr = xyz(C);
@@ -112,6 +116,12 @@ __global__ __launch_bounds__(256) void k_shade(ShadeArgs a) {
             }
             if (m.type_pad[0] == MCRT_MAT_EMISSIVE) {  // :608-610
                 radiance = m.ior_pad[0] * (thr * ld3(m.albedo_fuzz));
+                if (a.nee) {  // MIS weight of a BSDF-sampled emitter hit
+                    const float pb = a.P[slot];
+                    const uint32_t e =
+                        (pb > 0.0f && a.scene.n_emit) ? emitter_of(a.scene, __float_as_uint(H.w)) : kNone;
+                    if (e != kNone) radiance = power_heuristic(pb, light_pdf(a.scene, e, o, d, H.x)) * radiance;
+                }
             } else {
                 Scattered sc = scatter_material(m, d, s, rng);
                 if (!sc.absorbed) {
@@ -122,11 +132,37 @@ __global__ __launch_bounds__(256) void k_shade(ShadeArgs a) {
                         nA = make_float4(s.p.x, s.p.y, s.p.z, A.w);
                         nB = make_float4(sc.dir.x, sc.dir.y, sc.dir.z, __uint_as_float(rng));
                         nC = make_float4(thr.x, thr.y, thr.z, __uint_as_float(nb));
+                        const bool lambert = m.type_pad[0] != MCRT_MAT_METAL &&
+                                             m.type_pad[0] != MCRT_MAT_DIELECTRIC;
+                        if (a.nee && lambert) {
+                            n_pdf = fmaxf(dot(s.n, sc.dir), 0.0f) * kInvPi;
+                            LightSample ls;
+                            uint32_t lrng = pcg_hash(rng ^ kLightStream);
+                            if (a.scene.n_emit && sample_light(a.scene, s.p, lrng, ls)) {
+                                const float cosx = dot(s.n, ls.dir);
+                                if (cosx > 0.0f) {
+                                    // thr already holds thr * albedo.
+                                    const float wgt = power_heuristic(ls.pdf, cosx * kInvPi) * cosx * kInvPi / ls.pdf;
+                                    const V3 c = wgt * (thr * ls.Le);
+                                    shadow = true;
+                                    sO = make_float4(s.p.x, s.p.y, s.p.z, A.w);
+                                    sD = make_float4(ls.dir.x, ls.dir.y, ls.dir.z, ls.dist * (1.0f - 1e-3f));
+                                    sC = make_float4(c.x, c.y, c.z, 0.0f);
+                                }
+                            }
+                        }
                     }
                 }
             }
         }
-        if (done) a.samples[pid] = make_float4(radiance.x, radiance.y, radiance.z, 0.0f);
+        if (done) {
+            if (!a.nee) {
+                a.samples[pid] = make_float4(radiance.x, radiance.y, radiance.z, 0.0f);
+            } else if (radiance.x != 0.0f || radiance.y != 0.0f || radiance.z != 0.0f) {
+                float4 acc = a.samples[pid];
+                a.samples[pid] = make_float4(acc.x + radiance.x, acc.y + radiance.y, acc.z + radiance.z, 0.0f);
+            }
+        }
         alive = !done;
     }
     // Block-aggregated enqueue.
@@ -146,7 +182,55 @@ __global__ __launch_bounds__(256) void k_shade(ShadeArgs a) {
         a.A_out[o] = nA;
         a.B_out[o] = nB;
         a.C_out[o] = nC;
+        if (a.nee) a.P_out[o] = n_pdf;
     }
+    if (!a.nee) return;
+    // Shadow rays, block-aggregated the same way.
+    const uint64_t smask = __ballot(shadow);
+    if (__lane_id() == 0) s_swave[wave] = (uint32_t)__popcll(smask);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t total = s_swave[0] + s_swave[1] + s_swave[2] + s_swave[3];
+        s_sbase = total ? atomicAdd(a.shadow_count, total) : 0u;
+    }
+    __syncthreads();
+    if (shadow) {
+        uint32_t o = s_sbase;
+        for (uint32_t w = 0; w < wave; ++w) o += s_swave[w];
+        o += __builtin_amdgcn_mbcnt_hi((uint32_t)(smask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)smask, 0u));
+        a.S_o[o] = sO;
+        a.S_d[o] = sD;
+        a.S_c[o] = sC;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// k_occlude: next-event shadow rays; a visible light sample adds its
+// contribution to its path's sample (one shadow ray per path per bounce).
+// ---------------------------------------------------------------------------
+struct OccIO {
+    const OccArgs &a;
+    __device__ uint32_t count() const { return *a.count; }
+    __device__ uint32_t *work() const { return a.work; }
+    __device__ void load(uint32_t i, V3 &o, V3 &d, float &tmin, float &tmax) const {
+        const float4 so = a.S_o[i], sd = a.S_d[i];
+        o = xyz(so);
+        d = xyz(sd);
+        tmin = MCRT_T_MIN;
+        tmax = sd.w;
+    }
+    __device__ void store(uint32_t i, const Best &b) const {
+        if (b.prim != kNone) return;
+        const uint32_t pid = __float_as_uint(a.S_o[i].w);
+        const float4 c = a.S_c[i], s = a.samples[pid];
+        a.samples[pid] = make_float4(s.x + c.x, s.y + c.y, s.z + c.z, 0.0f);
+    }
+};
+
+template <int kBlock, bool kSpheres, bool kPostpone>
+__global__ __launch_bounds__(kBlock) void k_occlude(OccArgs a) {
+    __shared__ uint32_t lds[kShortStack * kBlock];
+    trace_persistent<kBlock, kSpheres, kPostpone>(a.scene, OccIO{a}, lds, a.ovf, a.ovf_stride);
 }
 
 // ---------------------------------------------------------------------------
@@ -225,6 +309,10 @@ template <int kBlock, bool kSpheres, bool kPostpone>
 struct TraceK {
     static const void *fn() { return reinterpret_cast<const void *>(&k_trace<kBlock, kSpheres, kPostpone>); }
 };
+template <int kBlock, bool kSpheres, bool kPostpone>
+struct OccludeK {
+    static const void *fn() { return reinterpret_cast<const void *>(&k_occlude<kBlock, kSpheres, kPostpone>); }
+};
 
 template <template <int, bool, bool> class K>
 const void *traversal_kernel(const DevScene &sc) {
@@ -259,12 +347,18 @@ hipError_t launch_trace(const TraceArgs &a, uint32_t grid, hipStream_t s) {
     return launch_traversal(traversal_kernel<TraceK>(a.scene), a, grid, s);
 }
 
+hipError_t launch_occlude(const OccArgs &a, uint32_t grid, hipStream_t s) {
+    return launch_traversal(traversal_kernel<OccludeK>(a.scene), a, grid, s);
+}
+
 hipError_t occupancy(int which, const DevScene &sc, int *blocks_per_cu) {
     const void *f = nullptr;
     if (which == 0)
         f = traversal_kernel<ExtendK>(sc);
     else if (which == 1)
         f = reinterpret_cast<const void *>(&k_shade);
+    else if (which == 3)
+        f = traversal_kernel<OccludeK>(sc);
     else
         f = traversal_kernel<TraceK>(sc);
     return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, f, which == 1 ? 256 : kTraceBlock, 0);

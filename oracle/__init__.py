@@ -139,6 +139,18 @@ def render(scene: Scene, camera, width, height, spp, first_frame=0, max_depth=8,
     return out, gb
 
 
+def render_nee(scene: Scene, camera, width, height, spp, first_frame=0, max_depth=8, threads=None):
+    """oracle_render with next-event estimation + MIS; (accum, gbuf)."""
+    L = lib()
+    L.oracle_render_nee.argtypes = [C.c_void_p, C.c_void_p] + [C.c_uint32] * 5 + [C.c_void_p] * 3 + [C.c_int]
+    cam = np.ascontiguousarray(camera)
+    out = np.empty((height, width, 4), np.float32)
+    gb = np.empty((height, width, 4), np.float32)
+    L.oracle_render_nee(scene.ref(), _p(cam), width, height, spp, first_frame, max_depth, None, _p(out),
+                        _p(gb), threads or default_threads())
+    return out, gb
+
+
 def sample(scene: Scene, camera, width, height, x, y, frame, max_depth=8):
     L = lib()
     cam = np.ascontiguousarray(camera)

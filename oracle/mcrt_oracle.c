@@ -166,7 +166,8 @@ typedef struct {
     float u, v;
     int front;
     uint32_t mat;
-    uint32_t id; /* primitive id as reported by the closest-hit query */
+    uint32_t id;   /* primitive id as reported by the closest-hit query */
+    uint32_t prim; /* leaf-order index; bit 31 marks a sphere */
 } Hit;
 
 static Hit sphere_hit(v3 o, v3 d, const GpuSphere *sp, float t_min, float t_max) {
@@ -252,6 +253,7 @@ static int tri_candidate(const OracleScene *s, uint32_t ti, v3 o, v3 d,
     out->u = w * va->uv[0] + u * vb->uv[0] + v * vc->uv[0];
     out->v = w * va->uv[1] + u * vb->uv[1] + v * vc->uv[1];
     out->id = s->tri_orig ? s->tri_orig[ti] : ti;
+    out->prim = ti;
     return 1;
 }
 
@@ -302,6 +304,7 @@ static Hit sphere_bvh_hit_ref(const OracleScene *s, v3 o, v3 d, float t_min,
             Hit h = sphere_hit(o, d, &s->spheres[si], t_min, t_max);
             if (h.t > 0.0f) {
                 h.id = MCRT_ID_SPHERE_BIT | (s->sphere_orig ? s->sphere_orig[si] : si);
+                h.prim = 0x80000000u | si;
                 best = h;
                 t_max = h.t;
             }
@@ -395,6 +398,7 @@ static Hit sphere_exact(const OracleScene *s, v3 o, v3 d, float t_min, float t_m
                 Hit h = sphere_hit(o, d, &s->spheres[si], t_min, t_max);
                 if (h.t > 0.0f) {
                     h.id = MCRT_ID_SPHERE_BIT | (s->sphere_orig ? s->sphere_orig[si] : si);
+                    h.prim = 0x80000000u | si;
                     if (best.t < 0.0f || better(h.t, h.id, best.t, best.id)) {
                         best = h;
                         bt = h.t;
@@ -600,11 +604,242 @@ static v3 path_color(const OracleScene *s, v3 o, v3 d, uint32_t *rng, uint32_t m
     return mk(0.0f, 0.0f, 0.0f);
 }
 
+/* ------------------------------------------------------------------------ */
+/* Next-event estimation: the build's opt-in estimator (no reference        */
+/* counterpart; the shader reserves binding 12, src/gpu_layout.rs:51-52,    */
+/* for it).  Emitter chosen with probability proportional to area x         */
+/* luminance; triangle points uniform in area; sphere directions uniform in */
+/* the subtended cone; power heuristic against the cosine BSDF sampling;    */
+/* light draws from their own PCG stream hashed from the path state after   */
+/* the scatter.  An independent CPU statement of that estimator, in the     */
+/* same operation order as the kernels.                                     */
+/* ------------------------------------------------------------------------ */
+#define INV_PI 0.31830988618379067f
+#define LIGHT_STREAM 0x68E31DA4u
+#define EMIT_SPHERE 0x80000000u
+
+typedef struct {
+    uint32_t n;
+    float *cdf, *prob, *area;
+    uint32_t *ref;
+    v3 *le;
+    uint32_t *tri_emit, *sph_emit;
+} Emitters;
+
+static float lum3(v3 c) { return 0.2126f * c.x + 0.7152f * c.y + 0.0722f * c.z; }
+
+static int emission_of(const OracleScene *s, uint32_t mat, v3 *le) {
+    const GpuMaterial *g = &s->mats->materials[mat < MAX_MAT_IDX ? mat : MAX_MAT_IDX];
+    if (g->type_pad[0] != MCRT_MAT_EMISSIVE) return 0;
+    *le = mk(g->ior_pad[0] * g->albedo_fuzz[0], g->ior_pad[0] * g->albedo_fuzz[1],
+             g->ior_pad[0] * g->albedo_fuzz[2]);
+    return 1;
+}
+
+static void emitters_free(Emitters *E) {
+    free(E->cdf); free(E->prob); free(E->area); free(E->ref); free(E->le);
+    free(E->tri_emit); free(E->sph_emit);
+    memset(E, 0, sizeof *E);
+}
+
+static void emitters_build(const OracleScene *s, Emitters *E) {
+    memset(E, 0, sizeof *E);
+    uint32_t cap = s->n_tris + s->n_spheres;
+    float *w = (float *)malloc(sizeof(float) * (cap ? cap : 1));
+    E->ref = (uint32_t *)malloc(sizeof(uint32_t) * (cap ? cap : 1));
+    E->area = (float *)malloc(sizeof(float) * (cap ? cap : 1));
+    E->le = (v3 *)malloc(sizeof(v3) * (cap ? cap : 1));
+    uint32_t n = 0;
+    for (uint32_t i = 0; i < s->n_tris; ++i) {
+        v3 le;
+        if (!emission_of(s, s->tris[i].mat_idx & 0x7FFFFFFFu, &le)) continue;
+        v3 p0 = tri_pos(s, i, 0);
+        v3 cr = cross(sub(tri_pos(s, i, 1), p0), sub(tri_pos(s, i, 2), p0));
+        float area = 0.5f * sqrtf(dot(cr, cr));
+        float wt = area * lum3(le);
+        if (!(wt > 0.0f) || !isfinite(wt)) continue;
+        E->ref[n] = i; E->area[n] = area; E->le[n] = le; w[n] = wt; ++n;
+    }
+    for (uint32_t i = 0; i < s->n_spheres; ++i) {
+        v3 le;
+        if (!emission_of(s, s->spheres[i].mat_and_pad[0], &le)) continue;
+        float r = s->spheres[i].center_r[3];
+        float area = 4.0f * 3.14159265358979f * r * r;
+        float wt = area * lum3(le);
+        if (!(wt > 0.0f) || !isfinite(wt)) continue;
+        E->ref[n] = EMIT_SPHERE | i; E->area[n] = area; E->le[n] = le; w[n] = wt; ++n;
+    }
+    E->n = n;
+    E->cdf = (float *)malloc(sizeof(float) * (n ? n : 1));
+    E->prob = (float *)malloc(sizeof(float) * (n ? n : 1));
+    E->tri_emit = (uint32_t *)malloc(sizeof(uint32_t) * (s->n_tris ? s->n_tris : 1));
+    E->sph_emit = (uint32_t *)malloc(sizeof(uint32_t) * (s->n_spheres ? s->n_spheres : 1));
+    for (uint32_t i = 0; i < s->n_tris; ++i) E->tri_emit[i] = 0xFFFFFFFFu;
+    for (uint32_t i = 0; i < s->n_spheres; ++i) E->sph_emit[i] = 0xFFFFFFFFu;
+    double total = 0.0, run = 0.0;
+    for (uint32_t k = 0; k < n; ++k) total += w[k];
+    for (uint32_t k = 0; k < n; ++k) {
+        E->prob[k] = (float)(w[k] / total);
+        run += w[k];
+        E->cdf[k] = k + 1 == n ? 1.0f : (float)(run / total);
+        if (E->ref[k] & EMIT_SPHERE) E->sph_emit[E->ref[k] & ~EMIT_SPHERE] = k;
+        else E->tri_emit[E->ref[k]] = k;
+    }
+    free(w);
+}
+
+static inline float power_heuristic(float a, float b) {
+    float a2 = a * a;
+    return a2 / (a2 + b * b);
+}
+
+static inline float cone_one_minus_cos(float r2, float d2) {
+    float s2 = r2 / d2;
+    return s2 / (1.0f + sqrtf(1.0f - s2));
+}
+
+/* Orthonormal basis (Duff et al., JCGT 6(1), 2017). */
+static void onb(v3 n, v3 *t1, v3 *t2) {
+    float sg = copysignf(1.0f, n.z);
+    float a = -1.0f / (sg + n.z);
+    float b = n.x * n.y * a;
+    *t1 = mk(1.0f + sg * n.x * n.x * a, sg * b, -sg * n.x);
+    *t2 = mk(b, sg + n.y * n.y * a, -n.y);
+}
+
+static float nee_light_pdf(const OracleScene *s, const Emitters *E, uint32_t e, v3 x, v3 d, float t) {
+    uint32_t ref = E->ref[e];
+    if (ref & EMIT_SPHERE) {
+        const GpuSphere *sp = &s->spheres[ref & ~EMIT_SPHERE];
+        v3 w = sub(ld3(sp->center_r), x);
+        float d2 = dot(w, w), r2 = sp->center_r[3] * sp->center_r[3];
+        if (!(d2 > r2)) return 0.0f;
+        return E->prob[e] / (TAU * cone_one_minus_cos(r2, d2));
+    }
+    v3 p0 = tri_pos(s, ref, 0);
+    v3 ng = cross(sub(tri_pos(s, ref, 1), p0), sub(tri_pos(s, ref, 2), p0));
+    float cosl = fabsf(dot(ng, d)) / sqrtf(dot(ng, ng));
+    if (!(cosl > 1e-7f)) return 0.0f;
+    return E->prob[e] * (t * t) / (E->area[e] * cosl);
+}
+
+static int nee_sample(const OracleScene *s, const Emitters *E, v3 x, uint32_t *lrng, v3 *dir,
+                      float *dist, v3 *le, float *pdf) {
+    float u0 = oracle_rand_f32(lrng), u1 = oracle_rand_f32(lrng), u2 = oracle_rand_f32(lrng);
+    uint32_t lo = 0, hi = E->n - 1u;
+    while (lo < hi) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (u0 < E->cdf[mid]) hi = mid; else lo = mid + 1u;
+    }
+    uint32_t e = lo, ref = E->ref[e];
+    *le = E->le[e];
+    if (ref & EMIT_SPHERE) {
+        const GpuSphere *sp = &s->spheres[ref & ~EMIT_SPHERE];
+        v3 w = sub(ld3(sp->center_r), x);
+        float d2 = dot(w, w), r2 = sp->center_r[3] * sp->center_r[3];
+        if (!(d2 > r2)) return 0;
+        float omc = cone_one_minus_cos(r2, d2);
+        float ct = 1.0f - u1 * omc;
+        float st = sqrtf(fmaxf(0.0f, 1.0f - ct * ct));
+        float phi = TAU * u2;
+        v3 wn = scl(1.0f / sqrtf(d2), w);
+        v3 t1, t2;
+        onb(wn, &t1, &t2);
+        *dir = normalize(add(add(scl(st * cosf(phi), t1), scl(st * sinf(phi), t2)), scl(ct, wn)));
+        float b = dot(*dir, w);
+        *dist = b - sqrtf(fmaxf(r2 - (d2 - b * b), 0.0f));
+        *pdf = E->prob[e] / (TAU * omc);
+        return *dist > 0.0f && *pdf > 0.0f && *pdf < 3.0e38f;
+    }
+    v3 v0 = tri_pos(s, ref, 0);
+    v3 e1 = sub(tri_pos(s, ref, 1), v0), e2 = sub(tri_pos(s, ref, 2), v0);
+    float su = sqrtf(u1);
+    float bu = su * (1.0f - u2), bv = su * u2;
+    v3 y = add(add(v0, scl(bu, e1)), scl(bv, e2));
+    v3 w = sub(y, x);
+    float d2 = dot(w, w);
+    if (!(d2 > 0.0f)) return 0;
+    *dist = sqrtf(d2);
+    *dir = scl(1.0f / *dist, w);
+    if (s->tris[ref].mat_idx >> 31u) {
+        /* a back-face-culled emitter is invisible from behind (:438-452) */
+        const GpuTriangle *tri = &s->tris[ref];
+        float bw = 1.0f - bu - bv;
+        v3 n = normalize(add(add(scl(bw, ld3(s->verts[tri->v[0]].normal)), scl(bu, ld3(s->verts[tri->v[1]].normal))),
+                             scl(bv, ld3(s->verts[tri->v[2]].normal))));
+        if (!(dot(*dir, n) < 0.0f)) return 0;
+    }
+    v3 ng = cross(e1, e2);
+    float cosl = fabsf(dot(ng, *dir)) / sqrtf(dot(ng, ng));
+    if (!(cosl > 1e-7f)) return 0;
+    *pdf = E->prob[e] * d2 / (E->area[e] * cosl);
+    return *pdf < 3.0e38f;
+}
+
+static v3 path_color_nee(const OracleScene *s, const Emitters *E, v3 o, v3 d, uint32_t *rng,
+                         uint32_t max_depth, float gbuf[4]) {
+    v3 thr = mk(1.0f, 1.0f, 1.0f), L = mk(0.0f, 0.0f, 0.0f);
+    float pdf_prev = 0.0f; /* BSDF pdf of the arriving ray; 0 after the camera or a specular event */
+    int first = 1;
+    for (uint32_t b = 0; b < max_depth; ++b) {
+        Hit h = world_exact(s, o, d, MCRT_T_MIN, MCRT_T_MAX);
+        if (h.t < 0.0f) {
+            if (first) { gbuf[0] = 0.0f; gbuf[1] = 0.0f; gbuf[2] = 0.0f; gbuf[3] = MCRT_T_MAX; }
+            return add(L, mul(thr, sample_env(s, d)));
+        }
+        if (first) {
+            gbuf[0] = h.n.x; gbuf[1] = h.n.y; gbuf[2] = h.n.z; gbuf[3] = h.t;
+            first = 0;
+        }
+        uint32_t mi = h.mat < MAX_MAT_IDX ? h.mat : MAX_MAT_IDX;
+        const GpuMaterial *m = &s->mats->materials[mi];
+        if (m->type_pad[0] == MCRT_MAT_EMISSIVE) {
+            v3 rad = scl(m->ior_pad[0], mul(thr, ld3(m->albedo_fuzz)));
+            if (pdf_prev > 0.0f && E->n) {
+                uint32_t e = (h.prim & EMIT_SPHERE) ? E->sph_emit[h.prim & ~EMIT_SPHERE] : E->tri_emit[h.prim];
+                if (e != 0xFFFFFFFFu) rad = scl(power_heuristic(pdf_prev, nee_light_pdf(s, E, e, o, d, h.t)), rad);
+            }
+            return add(L, rad);
+        }
+        Scatter sr = scatter(m, d, &h, rng);
+        if (sr.absorbed) return L;
+        thr = mul(thr, sr.att);
+        if (b + 1u >= max_depth) return L;
+        pdf_prev = 0.0f;
+        if (m->type_pad[0] != MCRT_MAT_METAL && m->type_pad[0] != MCRT_MAT_DIELECTRIC) {
+            pdf_prev = fmaxf(dot(h.n, sr.dir), 0.0f) * INV_PI;
+            uint32_t lrng = oracle_pcg_hash(*rng ^ LIGHT_STREAM);
+            v3 ldir, le;
+            float dist, pdf;
+            if (E->n && nee_sample(s, E, h.p, &lrng, &ldir, &dist, &le, &pdf)) {
+                float cosx = dot(h.n, ldir);
+                if (cosx > 0.0f) {
+                    float wgt = power_heuristic(pdf, cosx * INV_PI) * cosx * INV_PI / pdf;
+                    v3 c = scl(wgt, mul(thr, le));
+                    if (world_exact(s, h.p, ldir, MCRT_T_MIN, dist * (1.0f - 1e-3f)).t < 0.0f) L = add(L, c);
+                }
+            }
+        }
+        o = h.p;
+        d = sr.dir;
+    }
+    return L;
+}
+
+static v3 camera_sample_ex(const OracleScene *s, const Emitters *E, const CameraUniform *cam,
+                           uint32_t W, uint32_t H, uint32_t x, uint32_t y, uint32_t frame,
+                           uint32_t max_depth, float gbuf[4], OracleCounts *c, uint64_t *segments);
+
 static v3 camera_sample(const OracleScene *s, const CameraUniform *cam, uint32_t W,
                         uint32_t H, uint32_t x, uint32_t y, uint32_t frame,
                         uint32_t max_depth, float gbuf[4], OracleCounts *c,
                         uint64_t *segments) {
-    (void)s;
+    return camera_sample_ex(s, NULL, cam, W, H, x, y, frame, max_depth, gbuf, c, segments);
+}
+
+static v3 camera_sample_ex(const OracleScene *s, const Emitters *E, const CameraUniform *cam,
+                           uint32_t W, uint32_t H, uint32_t x, uint32_t y, uint32_t frame,
+                           uint32_t max_depth, float gbuf[4], OracleCounts *c, uint64_t *segments) {
     uint32_t rng = oracle_pixel_seed(x, y, W, frame);
     float u = ((float)x + oracle_rand_f32(&rng)) / (float)W;
     float v = 1.0f - ((float)y + oracle_rand_f32(&rng)) / (float)H;
@@ -616,6 +851,7 @@ static v3 camera_sample(const OracleScene *s, const CameraUniform *cam, uint32_t
     v3 ro = add(ld3(cam->origin), lens);
     v3 rd = normalize(sub(focal, ro));
     /* `mix(prev, s, w)` of the caller follows; the pixel's sample only here. */
+    if (E) return path_color_nee(s, E, ro, rd, &rng, max_depth, gbuf);
     return path_color(s, ro, rd, &rng, max_depth, gbuf, c, segments);
 }
 
@@ -641,6 +877,7 @@ typedef struct {
     OracleCounts counts;
     uint64_t segments;
     int want_counts;
+    const Emitters *E; /* next-event estimation, or NULL */
 } RenderJob;
 
 static void *render_worker(void *arg) {
@@ -659,9 +896,9 @@ static void *render_worker(void *arg) {
             float g[4] = {0, 0, 0, 0};
             for (uint32_t i = 0; i < j->spp; ++i) {
                 uint32_t f = j->first + i;
-                v3 smp = camera_sample(j->s, j->cam, j->W, j->H, x, y, f, j->depth, g,
-                                       j->want_counts ? &j->counts : NULL,
-                                       j->want_counts ? &j->segments : NULL);
+                v3 smp = camera_sample_ex(j->s, j->E, j->cam, j->W, j->H, x, y, f, j->depth, g,
+                                          j->want_counts ? &j->counts : NULL,
+                                          j->want_counts ? &j->segments : NULL);
                 uint32_t fc = f < MCRT_MAX_FRAME_COUNT ? f : MCRT_MAX_FRAME_COUNT;
                 float w = 1.0f / (float)(fc + 1u);
                 acc[0] = mixf(acc[0], smp.x, w);
@@ -678,10 +915,33 @@ static void *render_worker(void *arg) {
     return NULL;
 }
 
+static void render_run(const OracleScene *s, const Emitters *E, const CameraUniform *cam, uint32_t W,
+                       uint32_t H, uint32_t spp, uint32_t first_frame, uint32_t max_depth,
+                       const float *accum_in, float *out_accum, float *out_gbuf,
+                       OracleCounts *counts, uint64_t *segments, int n_threads);
+
 void oracle_render(const OracleScene *s, const CameraUniform *cam, uint32_t W, uint32_t H,
                    uint32_t spp, uint32_t first_frame, uint32_t max_depth,
                    const float *accum_in, float *out_accum, float *out_gbuf,
                    OracleCounts *counts, uint64_t *segments, int n_threads) {
+    render_run(s, NULL, cam, W, H, spp, first_frame, max_depth, accum_in, out_accum, out_gbuf, counts,
+               segments, n_threads);
+}
+
+void oracle_render_nee(const OracleScene *s, const CameraUniform *cam, uint32_t W, uint32_t H,
+                       uint32_t spp, uint32_t first_frame, uint32_t max_depth,
+                       const float *accum_in, float *out_accum, float *out_gbuf, int n_threads) {
+    Emitters E;
+    emitters_build(s, &E);
+    render_run(s, &E, cam, W, H, spp, first_frame, max_depth, accum_in, out_accum, out_gbuf, NULL, NULL,
+               n_threads);
+    emitters_free(&E);
+}
+
+static void render_run(const OracleScene *s, const Emitters *E, const CameraUniform *cam, uint32_t W,
+                       uint32_t H, uint32_t spp, uint32_t first_frame, uint32_t max_depth,
+                       const float *accum_in, float *out_accum, float *out_gbuf,
+                       OracleCounts *counts, uint64_t *segments, int n_threads) {
     if (n_threads < 1) n_threads = 1;
     RenderJob *jobs = (RenderJob *)calloc((size_t)n_threads, sizeof(RenderJob));
     pthread_t *th = (pthread_t *)calloc((size_t)n_threads, sizeof(pthread_t));
@@ -692,6 +952,7 @@ void oracle_render(const OracleScene *s, const CameraUniform *cam, uint32_t W, u
         j->accum_in = accum_in; j->accum = out_accum; j->gbuf = out_gbuf;
         j->tid = t; j->nthreads = n_threads;
         j->want_counts = counts != NULL || segments != NULL;
+        j->E = E;
         pthread_create(&th[t], NULL, render_worker, j);
     }
     if (counts) memset(counts, 0, sizeof *counts);

@@ -93,6 +93,17 @@ void oracle_render(const OracleScene *s, const CameraUniform *cam,
                    const float *accum_in, float *out_accum, float *out_gbuf,
                    OracleCounts *counts, uint64_t *segments, int n_threads);
 
+/* oracle_render with the build's opt-in next-event estimator (emitters =
+ * emissive triangles and spheres, chosen by area x luminance; power-heuristic
+ * MIS against the cosine BSDF sampling; light draws from a separate PCG
+ * stream).  No reference counterpart; its expectation is the reference
+ * estimator's. */
+void oracle_render_nee(const OracleScene *s, const CameraUniform *cam,
+                       uint32_t width, uint32_t height, uint32_t spp,
+                       uint32_t first_frame, uint32_t max_depth,
+                       const float *accum_in, float *out_accum, float *out_gbuf,
+                       int n_threads);
+
 /* One sample of one pixel (path_color of cs_main) with its G-buffer texel. */
 void oracle_sample(const OracleScene *s, const CameraUniform *cam,
                    uint32_t width, uint32_t height, uint32_t x, uint32_t y,
