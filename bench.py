@@ -262,8 +262,11 @@ def denoise_eval(ctx):
         return {"skipped": "missing " + ", ".join(os.path.relpath(p, ROOT) for p in missing)}
     log("denoise eval: 1080p icosphere, learned vs bilateral")
     dn = learned.load_trained(ctx)
-    res = learned.evaluate(ctx, dn, reps=3, log=log)
+    dn_nee = learned.load_trained(ctx, learned.WEIGHTS_NEE_PATH) if os.path.exists(learned.WEIGHTS_NEE_PATH) else None
+    res = learned.evaluate(ctx, dn, reps=3, log=log, dn_nee=dn_nee)
     dn.close()
+    if dn_nee is not None:
+        dn_nee.close()
     res["infer_mfma"] = {"bound": "mfma", "achieved": res["infer_tflops"], "peak": MFMA_BF16_PEAK_TFLOPS,
                          "unit": "TFLOP/s", "frac": round(res["infer_tflops"] / MFMA_BF16_PEAK_TFLOPS, 4)}
     return res

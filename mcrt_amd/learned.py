@@ -39,6 +39,7 @@ from ._lib import Context, McrtError, MCRT_OK, load
 N_PARAMS = 40771
 PKG_DIR = os.path.dirname(os.path.abspath(__file__))
 WEIGHTS_PATH = os.path.join(PKG_DIR, "weights", "denoiser_c4.npy")
+WEIGHTS_NEE_PATH = os.path.join(PKG_DIR, "weights", "denoiser_c4_nee.npy")  # trained on NEE renders
 REFERENCE_CROPS = os.path.join(os.path.dirname(PKG_DIR), "tests", "golden", "c4_reference_crops.npz")
 EVAL_W, EVAL_H, EVAL_DEPTH = 1920, 1080, 8
 INFER_FLOP_PER_PIXEL = 81216  # 2 * 9 * (10*32 + 4*32*32 + 32*3) multiply-adds of the six convs
@@ -171,7 +172,17 @@ def reference_render(ctx: Context, cam, W: int, H: int, spp_in: int, spp_ref: in
 
 
 def render_pairs(ctx: Context, n_variants: int, size: int = 256, spp_in: int = 4, spp_ref: int = 4096,
-                 depth: int = 8, seed0: int = 1000, log=None) -> Pairs:
+                 depth: int = 8, seed0: int = 1000, log=None, nee: bool = False) -> Pairs:
+    """Training pairs; with `nee`, input and reference are both rendered
+    with next-event estimation (same expectation, lower noise)."""
+    ctx.set_nee(nee)
+    try:
+        return _render_pairs(ctx, n_variants, size, spp_in, spp_ref, depth, seed0, log)
+    finally:
+        ctx.set_nee(False)
+
+
+def _render_pairs(ctx, n_variants, size, spp_in, spp_ref, depth, seed0, log) -> Pairs:
     import torch
 
     dev = torch.device("cuda:0")
@@ -276,7 +287,17 @@ def time_to_rel_mse(ctx: Context, wl, method: str, spps, crops, origins, dn: Den
     """Sweep spp upward; at each, time render (+ denoise) over `reps` runs
     (median, inputs resident, scene uploaded) and take relMSE on the
     reference windows.  Stops at the first spp at or below `threshold`.
-    method: "none", "bilateral" or "learned"."""
+    method: "none", "bilateral" or "learned", optionally prefixed "nee_"
+    (or "nee" alone) for renders with next-event estimation."""
+    ctx.set_nee(method.startswith("nee"))
+    try:
+        return _time_to_rel_mse(ctx, wl, method.replace("nee_", "").replace("nee", "none"), spps, crops,
+                                origins, dn, threshold, reps, log, method)
+    finally:
+        ctx.set_nee(False)
+
+
+def _time_to_rel_mse(ctx, wl, method, spps, crops, origins, dn, threshold, reps, log, label):
     import torch
 
     dev = torch.device("cuda:0")
@@ -308,10 +329,17 @@ def time_to_rel_mse(ctx: Context, wl, method: str, spps, crops, origins, dn: Den
         row = {"spp": spp, "ms": round(float(np.median(times)) * 1e3, 3), "relmse": round(err, 6)}
         rows.append(row)
         if log:
-            log(f"{method} spp {spp}: {row['ms']:.2f} ms relMSE {err:.4e}")
+            log(f"{label} spp {spp}: {row['ms']:.2f} ms relMSE {err:.4e}")
         if err <= threshold:
             hit = row
             break
+        # A method whose error has stopped falling (a biased denoiser) will
+        # not reach the threshold: stop when the last three points improve
+        # on the best before them by less than 10%.
+        if len(rows) >= 6:
+            before = min(r["relmse"] for r in rows[:-3])
+            if min(r["relmse"] for r in rows[-3:]) > 0.9 * before:
+                break
     return {"reached": hit, "sweep": rows}
 
 
@@ -336,7 +364,9 @@ def time_infer(dn: Denoiser, W: int, H: int, reps: int = 10) -> float:
     return float(np.median(ts))
 
 
-def evaluate(ctx: Context, dn: Denoiser, reps: int = 3, log=None) -> dict:
+def evaluate(ctx: Context, dn: Denoiser, reps: int = 3, log=None, dn_nee: Denoiser = None) -> dict:
+    """The sweep for every method; `dn_nee` (weights trained on renders with
+    next-event estimation) denoises the NEE renders when given."""
     crops, origins, ref_spp = load_reference_crops()
     wl = eval_workload()
     ctx.upload_scene(wl.buffers)
@@ -346,8 +376,12 @@ def evaluate(ctx: Context, dn: Denoiser, reps: int = 3, log=None) -> dict:
     res = {"frame": f"icosphere_c2 {EVAL_W}x{EVAL_H} depth {EVAL_DEPTH}",
            "reference": f"{len(origins)} windows {crops.shape[1]}x{crops.shape[2]}, {ref_spp} spp FP64 oracle",
            "threshold": 1e-2}
-    for method in ("learned", "bilateral", "none"):
-        res[method] = time_to_rel_mse(ctx, wl, method, spps, crops, origins, dn=dn, reps=reps, log=log)
+    # The metric's comparison (plain renders), then the same with next-event
+    # estimation.
+    for method in ("learned", "bilateral", "none", "nee_learned", "nee_bilateral", "nee"):
+        net = dn_nee if (method == "nee_learned" and dn_nee is not None) else dn
+        res[method] = time_to_rel_mse(ctx, wl, method, spps, crops, origins, dn=net, reps=reps, log=log)
+    res["nee_learned_weights"] = "trained on NEE renders" if dn_nee is not None else "trained on plain renders"
     t_inf = time_infer(dn, EVAL_W, EVAL_H)
     res["infer_ms"] = round(t_inf * 1e3, 3)
     res["infer_tflops"] = round(INFER_FLOP_PER_PIXEL * EVAL_W * EVAL_H / t_inf / 1e12, 2)
@@ -366,15 +400,17 @@ def _main(argv=None):
     t.add_argument("--variants", type=int, default=512)
     t.add_argument("--spp-in", type=int, default=4)
     t.add_argument("--spp-ref", type=int, default=4096)
-    t.add_argument("--steps", type=int, default=4000)
+    t.add_argument("--steps", type=int, default=16000)
     t.add_argument("--batch", type=int, default=16)
     t.add_argument("--lr", type=float, default=1e-3)
     t.add_argument("--seed", type=int, default=0)
     t.add_argument("--loss", choices=sorted(Denoiser.LOSSES), default="rel_mse")
     t.add_argument("--warm-steps", type=int, default=2000, help="log-L2 steps before --loss")
-    t.add_argument("--out", default=WEIGHTS_PATH)
+    t.add_argument("--nee", action="store_true", help="render inputs and references with next-event estimation")
+    t.add_argument("--out", default=None, help="default: the package's weights file for the estimator")
     e = sub.add_parser("eval")
     e.add_argument("--weights", default=WEIGHTS_PATH)
+    e.add_argument("--weights-nee", default=WEIGHTS_NEE_PATH)
     e.add_argument("--reps", type=int, default=3)
     a = ap.parse_args(argv)
 
@@ -385,7 +421,8 @@ def _main(argv=None):
     with Context(0) as ctx:
         if a.cmd == "train":
             t0 = time.time()
-            pairs = render_pairs(ctx, a.variants, spp_in=a.spp_in, spp_ref=a.spp_ref, log=log)
+            out = a.out or (WEIGHTS_NEE_PATH if a.nee else WEIGHTS_PATH)
+            pairs = render_pairs(ctx, a.variants, spp_in=a.spp_in, spp_ref=a.spp_ref, log=log, nee=a.nee)
             t_pairs = time.time() - t0
             log(f"pairs rendered in {t_pairs:.1f} s")
             dn = Denoiser(ctx, seed=a.seed)
@@ -395,15 +432,16 @@ def _main(argv=None):
             torch.cuda.synchronize()
             t_train = time.time() - t1
             log(f"trained {a.steps} steps in {t_train:.1f} s")
-            save_weights(dn, a.out, {"variants": a.variants, "size": 256, "spp_in": a.spp_in,
-                                     "spp_ref": a.spp_ref, "steps": a.steps, "batch": a.batch, "lr": a.lr,
-                                     "seed": a.seed, "loss": a.loss, "warm_steps": a.warm_steps,
-                                     "loss_first": losses[0], "loss_last": losses[-1],
-                                     "pairs_s": round(t_pairs, 1), "train_s": round(t_train, 1)})
-            log(f"weights -> {a.out}")
+            save_weights(dn, out, {"variants": a.variants, "size": 256, "spp_in": a.spp_in,
+                                   "spp_ref": a.spp_ref, "nee": a.nee, "steps": a.steps, "batch": a.batch,
+                                   "lr": a.lr, "seed": a.seed, "loss": a.loss, "warm_steps": a.warm_steps,
+                                   "loss_first": losses[0], "loss_last": losses[-1],
+                                   "pairs_s": round(t_pairs, 1), "train_s": round(t_train, 1)})
+            log(f"weights -> {out}")
         else:
             dn = load_trained(ctx, a.weights)
-            print(json.dumps(evaluate(ctx, dn, reps=a.reps, log=log)), flush=True)
+            dn_nee = load_trained(ctx, a.weights_nee) if os.path.exists(a.weights_nee) else None
+            print(json.dumps(evaluate(ctx, dn, reps=a.reps, log=log, dn_nee=dn_nee)), flush=True)
 
 
 if __name__ == "__main__":
