@@ -179,6 +179,33 @@ def test_committed_weights_reach_the_target_on_the_1080p_frame(ctx):
     assert e_l <= 1e-2 and e_l < e_b < e_n
 
 
+def test_nee_weights_reach_the_target_at_one_spp(ctx):
+    # Weights trained on next-event-estimation renders, on a 1-spp NEE render
+    # of the 1920 x 1080 icosphere frame: relMSE on the reference windows at
+    # most 1e-2 (deterministic input).
+    import torch
+
+    crops, origins, _ = learned.load_reference_crops()
+    wl = learned.eval_workload()
+    ctx.upload_scene(wl.buffers)
+    dev = torch.device("cuda:0")
+    W, H = wl.width, wl.height
+    acc = torch.empty((H, W, 4), device=dev)
+    gb, al, lo = (torch.empty_like(acc) for _ in range(3))
+    s = torch.cuda.current_stream(dev).cuda_stream
+    ctx.set_nee(True)
+    try:
+        ctx.render_aux_dev(wl.camera, W, H, 1, 0, wl.depth, None, acc.data_ptr(), gb.data_ptr(), al.data_ptr(), s)
+    finally:
+        ctx.set_nee(False)
+    dn = learned.load_trained(ctx, learned.WEIGHTS_NEE_PATH)
+    dn.infer(acc[None], al[None], gb[None], lo[None], s)
+    torch.cuda.synchronize()
+    e = learned.crops_rel_mse(lo.cpu().numpy(), crops, origins)
+    print(f"relMSE at 1 spp with NEE + learned: {e:.4e}")
+    assert e <= 1e-2
+
+
 def test_training_reduces_loss(ctx, batch):
     noisy, albedo, gbuf, ref = batch
     dn = learned.Denoiser(ctx, seed=9)
