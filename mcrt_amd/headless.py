@@ -6,8 +6,9 @@ CLI > ``[render]`` of the ``<scene>.test.toml`` sidecar > defaults 512 x 512 x
 64 (src/headless.rs:36-40, :120-127), loads the scene YAML, builds the BVHs,
 renders frames ``0..spp-1`` on the GPU through libmcrt_hip.so, sanitises
 non-finite values, tonemaps (Reinhard + gamma 2.2, src/headless.rs:93-98) and
-writes an RGBA8 PNG.  ``--denoise bilateral`` applies the bilateral filter
-before tonemapping (an extension; the reference's headless path has none).
+writes an RGBA8 PNG.  Extensions (the reference's headless path has none):
+``--denoise bilateral|learned`` applies the bilateral filter or the learned
+denoiser before tonemapping; ``--nee`` renders with next-event estimation.
 """
 from __future__ import annotations
 
@@ -71,7 +72,12 @@ def write_png(path: str, rgba: np.ndarray) -> None:
 
 def run(scene_path: str, output_path: str, width: Optional[int] = None, height: Optional[int] = None,
         spp: Optional[int] = None, denoise: Optional[str] = None, device: int = 0,
-        max_depth: int = T.MAX_BOUNCES) -> dict:
+        max_depth: int = T.MAX_BOUNCES, nee: bool = False) -> dict:
+    """``denoise``: None / "none", "bilateral" or "learned" (committed
+    weights; the set trained on next-event-estimation renders when ``nee``).
+    ``nee``: render with next-event estimation (an extension)."""
+    if denoise not in (None, "none", "bilateral", "learned"):
+        raise ValueError(f"unknown denoiser {denoise!r}")
     side = load_sidecar(scene_path)
     width = max(1, int(width if width is not None else side.get("width", DEFAULT_WIDTH)))
     height = max(1, int(height if height is not None else side.get("height", DEFAULT_HEIGHT)))
@@ -81,13 +87,35 @@ def run(scene_path: str, output_path: str, width: Optional[int] = None, height: 
     cam = cam_mod.camera_for(loaded, width, height)
     with Context(device) as ctx:
         ctx.upload_scene(sb)
-        accum, gbuf = ctx.render(cam, width, height, spp, 0, max_depth)
-        if denoise == "bilateral":
-            accum = ctx.denoise_bilateral(accum, gbuf)
-        elif denoise not in (None, "none"):
-            raise ValueError(f"unknown denoiser {denoise!r}")
+        ctx.set_nee(nee)
+        if denoise == "learned":
+            accum = _render_learned(ctx, cam, width, height, spp, max_depth, device, nee)
+        else:
+            accum, gbuf = ctx.render(cam, width, height, spp, 0, max_depth)
+            if denoise == "bilateral":
+                accum = ctx.denoise_bilateral(accum, gbuf)
     write_png(output_path, to_rgba8(accum))
     return {"width": width, "height": height, "spp": spp}
+
+
+def _render_learned(ctx, cam, width, height, spp, max_depth, device, nee):
+    import torch
+
+    from . import learned
+
+    dev = torch.device(f"cuda:{device}")
+    acc = torch.empty((height, width, 4), dtype=torch.float32, device=dev)
+    gb, al, out = (torch.empty_like(acc) for _ in range(3))
+    s = torch.cuda.current_stream(dev).cuda_stream
+    ctx.render_aux_dev(cam, width, height, spp, 0, max_depth, None, acc.data_ptr(), gb.data_ptr(),
+                       al.data_ptr(), s)
+    dn = learned.load_trained(ctx, learned.WEIGHTS_NEE_PATH if nee else learned.WEIGHTS_PATH)
+    try:
+        dn.infer(acc[None], al[None], gb[None], out[None], s)
+        torch.cuda.synchronize(dev)
+        return out.cpu().numpy()
+    finally:
+        dn.close()
 
 
 def get_next_after(args, flag):
@@ -122,7 +150,7 @@ def main(argv=None) -> int:
         return 1
     try:
         run(scene, output, _opt_u32(args, "--width"), _opt_u32(args, "--height"),
-            _opt_u32(args, "--spp"), get_next_after(args, "--denoise"))
+            _opt_u32(args, "--spp"), get_next_after(args, "--denoise"), nee="--nee" in args)
     except (McrtError, scene_mod.SceneError, OSError, ValueError) as e:
         sys.stderr.write(f"headless render failed: {e}\n")
         return 1

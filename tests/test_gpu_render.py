@@ -162,3 +162,30 @@ def test_bilateral_sky_passthrough_quirk(ctx):
     gb[..., 3] = 1e9
     out = ctx.denoise_bilateral(acc, gb)
     assert np.array_equal(out[..., :3], acc[..., :3])
+
+
+def test_headless_cli_end_to_end(tmp_path):
+    """`python -m mcrt_amd --headless scene.yaml --output o.png ...` (the
+    mirror of src/main.rs:56-80): exit status, PNG size, the plain render
+    against the oracle's tonemapped image, and the denoiser / NEE extensions."""
+    from PIL import Image
+
+    scene = os.path.join(SCENES, "cornell_box_mini.yaml")
+    out = str(tmp_path / "plain.png")
+    assert headless.main(["--headless", scene, "--output", out, "--width", "48", "--height", "32",
+                          "--spp", "4"]) == 0
+    png = np.asarray(Image.open(out))
+    assert png.shape == (32, 48, 4)
+    loaded = scene_mod.load_scene_from_yaml(scene)
+    sb = scene_mod.build_scene_buffers(loaded)
+    ref, _ = oracle.render(oracle.Scene(sb), cam_mod.camera_for(loaded, 48, 32), 48, 32, 4, 0, 8)
+    r = regression.compare_images(png, headless.to_rgba8(ref))
+    assert r.mse <= 0.002 and r.psnr >= 38.0
+    box = os.path.join(SCENES, "emissive_box.yaml")
+    for extra in (["--denoise", "bilateral"], ["--denoise", "learned"], ["--nee"],
+                  ["--nee", "--denoise", "learned"]):
+        out = str(tmp_path / ("_".join(x.strip("-") for x in extra) + ".png"))
+        assert headless.main(["--headless", box, "--output", out, "--width", "40", "--height", "24",
+                              "--spp", "2"] + extra) == 0, extra
+        assert np.asarray(Image.open(out)).shape == (24, 40, 4)
+    assert headless.main(["--headless", box, "--output", str(tmp_path / "x.png"), "--denoise", "nope"]) == 1
