@@ -550,6 +550,11 @@ extern "C" int mcrt_render_aux_dev(mcrt_ctx *c, const CameraUniform *cam, uint32
                                    float *out_accum_dev, float *out_gbuf_dev,
                                    float *out_albedo_dev, void *stream);
 
+extern "C" int mcrt_traversal_mode(mcrt_ctx *c) {
+    if (!c || !c->has_scene) return -1;
+    return c->scene.postpone_leaves ? 1 : 0;
+}
+
 extern "C" int mcrt_set_nee(mcrt_ctx *c, int enabled) {
     if (!c) return MCRT_ERR_BAD_ARG;
     c->nee = enabled != 0;

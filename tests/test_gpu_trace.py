@@ -36,6 +36,7 @@ def test_c2_one_million_rays_bit_exact(ctx, c2):
     o, d = scenes.microbench_rays(1 << 20, lo, hi, seed=1234)
     print("trace c2 1M rays", flush=True)
     gid, gt, oid, ot, bad = _cmp(ctx, c2.buffers, o, d)
+    assert ctx.traversal_mode() == 1  # cache-resident: postponed leaves
     assert len(bad) == 0, f"{len(bad)} mismatches, first {bad[:5]}: gpu {gid[bad[:5]]} {gt[bad[:5]]} oracle {oid[bad[:5]]} {ot[bad[:5]]}"
     assert np.mean(gid != T.ID_MISS) > 0.5
 
@@ -121,3 +122,24 @@ def test_sixteen_million_rays_properties(ctx, c2):
     oid, ot = oracle.trace(oracle.Scene(c2.buffers), o[sel], d[sel], mode="exact")
     assert np.array_equal(gid[sel], oid)
     assert np.array_equal(gt[sel].view(np.uint32), ot.view(np.uint32))
+
+
+def test_c3_large_scene_bit_exact(ctx):
+    """The 2.16M-triangle voxel terrain spills the L2, so the library tests
+    leaves as it reaches them: ids and t must still be the oracle's, bit for
+    bit, for incoherent rays and for rays exactly parallel to the voxel faces
+    (zero direction components: the slab form's infinite inverse)."""
+    w = configs.c3(width=64, height=36)
+    lo, hi = configs.scene_aabb(w.buffers)
+    o, d = scenes.microbench_rays(1 << 18, lo, hi, seed=77)
+    rng = np.random.default_rng(5)
+    n_ax = 1 << 15
+    oa = (lo + rng.random((n_ax, 3)) * (hi - lo)).astype(np.float32)
+    axes = np.eye(3, dtype=np.float32)[rng.integers(0, 3, n_ax)] * rng.choice([-1.0, 1.0], (n_ax, 1)).astype(np.float32)
+    o = np.concatenate([o, oa])
+    d = np.concatenate([d, axes])
+    print("trace c3 large scene", flush=True)
+    gid, gt, oid, ot, bad = _cmp(ctx, w.buffers, o, d)
+    assert ctx.traversal_mode() == 0
+    assert len(bad) == 0, f"{len(bad)} mismatches, first {bad[:5]}: gpu {gid[bad[:5]]} {gt[bad[:5]]} oracle {oid[bad[:5]]} {ot[bad[:5]]}"
+    assert np.mean(gid != T.ID_MISS) > 0.1
