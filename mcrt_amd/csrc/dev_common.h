@@ -232,8 +232,13 @@ struct Stack {
         else
             ovf[(size_t)(i - kShortStack) * ovf_stride + gtid] = v;
     }
+    // The two reads go through explicitly LDS- and global-qualified pointers.
+    // Through plain pointers the compiler merges them into one flat load,
+    // which routes every LDS pop through the vector-memory path and makes each
+    // node step wait on both counters.
     __device__ __forceinline__ uint32_t get(uint32_t i) const {
-        return i < kShortStack ? lds[i * block + tid] : ovf[(size_t)(i - kShortStack) * ovf_stride + gtid];
+        if (i < kShortStack) return ((const __attribute__((address_space(3))) uint32_t *)lds)[i * block + tid];
+        return ((const __attribute__((address_space(1))) uint32_t *)ovf)[(size_t)(i - kShortStack) * ovf_stride + gtid];
     }
 };
 
