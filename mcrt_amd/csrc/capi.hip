@@ -271,6 +271,13 @@ inline float ubits(uint32_t u) {
     return f;
 }
 
+// Device counters per batch: path counts [max_depth+1], extend and shade work
+// counters [max_depth+1] each, then shadow-ray counts and occlusion work
+// counters [max_depth] each.
+uint32_t counters_per_batch(uint32_t max_depth) {
+    return 3u * (max_depth + 1u) + 2u * max_depth;
+}
+
 // Frames per batch: keep a batch at or below 2^24 paths.
 uint32_t frames_per_batch(uint32_t wh, uint32_t spp) {
     uint64_t cap = (1ull << 24) / std::max<uint32_t>(wh, 1);
@@ -607,9 +614,7 @@ extern "C" int mcrt_render_aux_dev(mcrt_ctx *c, const CameraUniform *cam, uint32
         c->cap_nee = cap;
     }
     const uint32_t n_batches = (spp + F - 1) / F;
-    // Per batch: path counts [max_depth+1], extend and shade work counters,
-    // then shadow-ray counts and occlusion work counters per bounce.
-    const uint32_t per_batch = 3u * (max_depth + 1u) + 2u * max_depth;
+    const uint32_t per_batch = counters_per_batch(max_depth);
     const size_t n_counters = (size_t)per_batch * n_batches;
     if ((rc = ensure(c, c->counters, n_counters * sizeof(uint32_t)))) return rc;
     uint32_t *counters = (uint32_t *)c->counters.p;
@@ -761,7 +766,7 @@ extern "C" int mcrt_render_counters(mcrt_ctx *c, uint64_t *segments, uint64_t *s
     if (rc) return rc;
     if (c->last_batches == 0) return fail(c, MCRT_ERR_NO_SCENE, "no render has run on this context");
     HIP_TRY(c, hipStreamSynchronize(c->last_stream));
-    const uint32_t per_batch = 3u * (c->last_depth + 1u);
+    const uint32_t per_batch = counters_per_batch(c->last_depth);
     std::vector<uint32_t> h((size_t)per_batch * c->last_batches);
     HIP_TRY(c, hipMemcpy(h.data(), c->counters.p, h.size() * 4, hipMemcpyDeviceToHost));
     uint64_t seg = 0;

@@ -141,6 +141,23 @@ def test_segment_counter_matches_oracle(ctx):
     assert abs(seg - cnt[3]) <= 1e-3 * cnt[3]
 
 
+def test_segment_counter_sums_over_batches(ctx):
+    """A render is split into batches of at most 2^24 paths: at 1024 x 1024,
+    17 spp runs as 16 + 1 frames, and its segment count must be the sum of
+    the two renders of those frame ranges."""
+    w = configs.c1(width=1024, height=1024)
+    ctx.upload_scene(w.buffers)
+    ctx.render(w.camera, 1024, 1024, 17, 0, 8, want_gbuf=False)
+    seg, smp = ctx.render_counters()
+    ctx.render(w.camera, 1024, 1024, 16, 0, 8, want_gbuf=False)
+    seg_a, _ = ctx.render_counters()
+    ctx.render(w.camera, 1024, 1024, 1, 16, 8, want_gbuf=False)
+    seg_b, _ = ctx.render_counters()
+    assert smp == 1024 * 1024 * 17
+    assert seg_a > 16 * 1024 * 1024 and seg_b > 1024 * 1024
+    assert seg == seg_a + seg_b
+
+
 def test_bilateral_matches_oracle(ctx):
     w = configs.c1(width=128, height=96)
     ctx.upload_scene(w.buffers)
