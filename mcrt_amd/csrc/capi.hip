@@ -645,9 +645,6 @@ extern "C" int mcrt_render_aux_dev(mcrt_ctx *c, const CameraUniform *cam, uint32
         g.wh = wh;
         g.frame0 = f0;
         g.n_paths = np;
-        // Scenes that spill the L2 trace faster from compact waves (strips of
-        // 4 rows); cache-resident ones from row order.
-        g.strip = c->scene.postpone_leaves ? 1u : 4u;
         g.A = (float4 *)c->A[0].p;
         g.B = (float4 *)c->B[0].p;
         g.C = (float4 *)c->C[0].p;

@@ -14,7 +14,6 @@ struct GenArgs {
     uint32_t width, height, wh;
     uint32_t frame0;   // frame of path 0 (frame_count of the first frame)
     uint32_t n_paths;  // wh * frames in this batch
-    uint32_t strip;    // queue order within a frame: strips of this many rows, column by column (1 = row-major)
     float4 *A, *B, *C;
     float4 *samples;   // next-event estimation: zeroed per path (else null)
     float *P;          // next-event estimation: BSDF pdf of the arriving ray, 0 = camera

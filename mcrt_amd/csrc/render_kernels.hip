@@ -29,20 +29,17 @@ namespace mcrt {
 // ---------------------------------------------------------------------------
 // k_generate: cs_main's seed, jitter and thin-lens ray (:633-652)
 // ---------------------------------------------------------------------------
-// Queue slot p holds frame p / wh; within a frame the slots walk strips of
-// a.strip rows column by column (4 rows: a wave covers 16 x 4 pixels), so a
-// wave's rays, and the bounce queues that keep their order, start close
-// together.  The path id keeps the row-major pixel index.
+// Queue order is pixel-major: the batch's frames of one pixel are adjacent,
+// so a wave starts from a few neighbouring pixels, and the bounce queues keep
+// that order through compaction.  The path id stays frame * wh + row-major
+// pixel.
 __global__ __launch_bounds__(256) void k_generate(GenArgs a) {
     uint32_t p = blockIdx.x * 256u + threadIdx.x;
     if (p >= a.n_paths) return;
-    const uint32_t i = p % a.wh;
-    const uint32_t fi = p / a.wh;
-    const uint32_t g = i / (a.strip * a.width);  // strip; all strips but the last are full
-    const uint32_t j = i - g * a.strip * a.width;
-    const uint32_t hg = min(a.strip, a.height - a.strip * g);
-    const uint32_t x = j / hg, y = a.strip * g + j % hg;
-    const uint32_t pid = fi * a.wh + y * a.width + x;
+    const uint32_t nf = a.n_paths / a.wh;  // frames in this batch
+    const uint32_t pix = p / nf, fi = p % nf;
+    const uint32_t x = pix % a.width, y = pix / a.width;
+    const uint32_t pid = fi * a.wh + pix;
     uint32_t rng;
     V3 ro, rd;
     camera_ray(a.cam, a.width, a.height, x, y, a.frame0 + fi, rng, ro, rd);

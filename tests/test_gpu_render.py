@@ -131,15 +131,14 @@ def test_voxel_terrain_c3_small(ctx, c3_small):
     np.testing.assert_allclose(den[..., :3], oden[..., :3], rtol=2e-5, atol=1e-5)
 
 
-def test_voxel_terrain_partial_last_strip(ctx):
-    """Scenes that spill the L2 queue their paths in strips of 4 rows; at 45
-    rows the last strip holds one row, and every pixel must still be rendered
-    exactly once (G-buffer and image against the oracle)."""
-    w = configs.c3(width=70, height=45, spp=1)
+def test_voxel_terrain_pixel_major_queue(ctx):
+    """Paths are queued pixel-major (a pixel's frames adjacent): at an odd
+    size with 3 frames per batch every (pixel, frame) path must still be
+    rendered exactly once (G-buffer and image against the oracle)."""
+    w = configs.c3(width=70, height=45, spp=3)
     ctx.upload_scene(w.buffers)
-    assert ctx.traversal_mode() == 0
-    acc, gb = ctx.render(w.camera, w.width, w.height, 1, 0, w.depth)
-    ref, rgb = oracle.render(oracle.Scene(w.buffers), w.camera, w.width, w.height, 1, 0, w.depth)
+    acc, gb = ctx.render(w.camera, w.width, w.height, 3, 0, w.depth)
+    ref, rgb = oracle.render(oracle.Scene(w.buffers), w.camera, w.width, w.height, 3, 0, w.depth)
     same_g, ok, r = _parity(acc, gb, ref, rgb)
     assert same_g >= 0.999 and ok >= 0.98, (same_g, ok)
 
