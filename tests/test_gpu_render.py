@@ -154,19 +154,19 @@ def test_segment_counter_matches_oracle(ctx):
 
 
 def test_segment_counter_sums_over_batches(ctx):
-    """A render is split into batches of at most 2^24 paths: at 1024 x 1024,
+    """A render is split into batches of at most 2^26 paths: at 2048 x 2048,
     17 spp runs as 16 + 1 frames, and its segment count must be the sum of
     the two renders of those frame ranges."""
-    w = configs.c1(width=1024, height=1024)
+    w = configs.c1(width=2048, height=2048)
     ctx.upload_scene(w.buffers)
-    ctx.render(w.camera, 1024, 1024, 17, 0, 8, want_gbuf=False)
+    ctx.render(w.camera, 2048, 2048, 17, 0, 8, want_gbuf=False)
     seg, smp = ctx.render_counters()
-    ctx.render(w.camera, 1024, 1024, 16, 0, 8, want_gbuf=False)
+    ctx.render(w.camera, 2048, 2048, 16, 0, 8, want_gbuf=False)
     seg_a, _ = ctx.render_counters()
-    ctx.render(w.camera, 1024, 1024, 1, 16, 8, want_gbuf=False)
+    ctx.render(w.camera, 2048, 2048, 1, 16, 8, want_gbuf=False)
     seg_b, _ = ctx.render_counters()
-    assert smp == 1024 * 1024 * 17
-    assert seg_a > 16 * 1024 * 1024 and seg_b > 1024 * 1024
+    assert smp == 2048 * 2048 * 17
+    assert seg_a > 16 * 2048 * 2048 and seg_b > 2048 * 2048
     assert seg == seg_a + seg_b
 
 
