@@ -211,13 +211,6 @@ int mcrt_render_dev(mcrt_ctx *ctx, const CameraUniform *camera,
  * use a separate PCG stream. */
 int mcrt_set_nee(mcrt_ctx *ctx, int enabled);
 
-/* Diagnostic, no reference counterpart: the closest-hit traversal variant
- * mcrt_upload_scene chose for the current scene from the size of its
- * traversal set (BVH nodes + primitives): 1 = postponed leaves (the set
- * fits the L2), 0 = leaves tested as reached (it spills the L2); -1 without a
- * scene.  Both variants return the same hits. */
-int mcrt_traversal_mode(mcrt_ctx *ctx);
-
 /* mcrt_render_dev plus a first-hit albedo buffer (W*H RGBA32F, last frame;
  * dielectric = 1, sky = 0).  No counterpart in the reference, whose G-buffer
  * holds normal and depth only: it is the extra input of the learned

@@ -78,7 +78,6 @@ def load() -> C.CDLL:
         "mcrt_build_sphere_bvh": (C.c_int, [_vp, _u32, _vp, C.POINTER(_u32), _vp, _vp]),
         "mcrt_set_profiling": (C.c_int, [_vp, C.c_int]),
         "mcrt_set_nee": (C.c_int, [_vp, C.c_int]),
-        "mcrt_traversal_mode": (C.c_int, [_vp]),
         "mcrt_kernel_stats": (C.c_int, [_vp, C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_uint64)]),
         "mcrt_reset_stats": (C.c_int, [_vp]),
         "mcrt_render_counters": (C.c_int, [_vp, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
@@ -116,8 +115,7 @@ def exported_symbols() -> list:
         "mcrt_create", "mcrt_destroy", "mcrt_last_error", "mcrt_upload_scene", "mcrt_render",
         "mcrt_render_dev", "mcrt_denoise_bilateral", "mcrt_denoise_bilateral_dev",
         "mcrt_trace_closest", "mcrt_trace_closest_dev", "mcrt_build_mesh_bvh",
-        "mcrt_build_sphere_bvh", "mcrt_set_profiling", "mcrt_set_nee", "mcrt_traversal_mode",
-        "mcrt_kernel_stats", "mcrt_reset_stats",
+        "mcrt_build_sphere_bvh", "mcrt_set_profiling", "mcrt_set_nee", "mcrt_kernel_stats", "mcrt_reset_stats",
         "mcrt_render_counters", "mcrt_mesh_voxel_chunks", "mcrt_diff_render", "mcrt_diff_render_dev",
         "mcrt_diff_adjoint", "mcrt_diff_adjoint_dev", "mcrt_render_aux_dev", "mcrt_denoiser_create",
         "mcrt_denoiser_destroy", "mcrt_denoiser_param_count", "mcrt_denoiser_get_weights",
@@ -341,11 +339,6 @@ class Context:
 
     def set_profiling(self, enabled: bool) -> None:
         self._check(self._lib.mcrt_set_profiling(self._h, 1 if enabled else 0))
-
-    def traversal_mode(self) -> int:
-        """Traversal variant chosen for the current scene: 1 = postponed
-        leaves (traversal set fits the L2), 0 = leaves tested as reached."""
-        return int(self._lib.mcrt_traversal_mode(self._h))
 
     def set_nee(self, enabled: bool) -> None:
         """Next-event estimation with MIS for later renders (default off)."""

@@ -27,10 +27,6 @@ namespace {
 
 thread_local std::string g_create_error;
 
-// Largest traversal working set (BVH nodes + triangles + spheres) for which
-// the postponed-leaf traversal loop is selected.
-constexpr size_t kPostponeMaxBytes = 32u << 20;
-
 struct DevBuf {
     void *p = nullptr;
     size_t bytes = 0;
@@ -540,14 +536,6 @@ extern "C" int mcrt_upload_scene(mcrt_ctx *c, const GpuSphere *spheres, uint32_t
     d.env_w = env_w;
     d.env_h = env_h;
     d.stack_depth = std::max(mt.depth, st.depth);
-    // Postponed leaves raise lane utilisation (+20% on a 70k-triangle scene)
-    // but fetch extra nodes; once the traversal data (nodes + triangles)
-    // outgrows the GPU's L2 capacity (8 x 4 MB) the extra fetches cost more
-    // than they save (-1% on a 2.16M-triangle scene), so they are used only
-    // below that size.
-    const size_t traversal_bytes = (mt.nodes.size() + st.nodes.size()) * sizeof(float4) +
-                                   (size_t)n_tris * 48 + (size_t)n_spheres * 32;
-    d.postpone_leaves = traversal_bytes <= kPostponeMaxBytes ? 1u : 0u;
     if (d.stack_depth > 96)
         return fail(c, MCRT_ERR_LIMIT, "BVH depth %u exceeds the traversal stack limit of 96",
                     d.stack_depth);
@@ -562,11 +550,6 @@ extern "C" int mcrt_render_aux_dev(mcrt_ctx *c, const CameraUniform *cam, uint32
                                    uint32_t max_depth, const float *accum_in_dev,
                                    float *out_accum_dev, float *out_gbuf_dev,
                                    float *out_albedo_dev, void *stream);
-
-extern "C" int mcrt_traversal_mode(mcrt_ctx *c) {
-    if (!c || !c->has_scene) return -1;
-    return c->scene.postpone_leaves ? 1 : 0;
-}
 
 extern "C" int mcrt_set_nee(mcrt_ctx *c, int enabled) {
     if (!c) return MCRT_ERR_BAD_ARG;

@@ -61,7 +61,6 @@ struct DevScene {
     const float4 *env;
     uint32_t env_w, env_h;
     uint32_t stack_depth;     // traversal stack entries needed per lane
-    uint32_t postpone_leaves; // host-side choice of the traversal loop variant
     // Emitters for next-event estimation: emissive triangles and spheres,
     // chosen with probability proportional to area x luminance.
     //   emit[2e] = (prob, kind | leaf-order index bits, area, -),
@@ -379,9 +378,8 @@ __device__ __forceinline__ Best trace_one(const DevScene &sc, V3 o, V3 d, float 
 
 // IO policy: count(), work(), load(slot, o, d, t_min, t_max), store(slot, best).
 // kSpheres = false compiles the variant for scenes without spheres (no
-// sphere phase state), launched when the sphere BVH is empty.  kPostpone
-// selects the postponed-leaf node loop (see below).
-template <int kBlock, bool kSpheres, bool kPostpone, class IO>
+// sphere phase state), launched when the sphere BVH is empty.
+template <int kBlock, bool kSpheres, class IO>
 __device__ __forceinline__ void trace_persistent(const DevScene &sc, const IO &io, uint32_t *lds,
                                                  uint32_t *ovf, uint32_t ovf_stride) {
     const uint32_t lane = __lane_id();
@@ -441,38 +439,29 @@ __device__ __forceinline__ void trace_persistent(const DevScene &sc, const IO &i
             continue;
         }
         // ---- internal nodes until every lane holds a leaf or is done -------
-        // kPostpone: a lane that reaches a leaf postpones it and keeps
-        // descending while other lanes still search for theirs; its extra
-        // steps cull with an older (larger) best t, so they only visit more
-        // nodes.  This raises lane utilisation where the scene is cache
-        // resident and costs extra fetches where it is not.
+        // A lane that reaches a leaf postpones it and keeps descending while
+        // other lanes still search for theirs; its extra steps cull with an
+        // older (larger) best t, so they only visit more nodes, and they keep
+        // the lane busy instead of idle in the wave.
         for (;;) {
-            if (kPostpone && slot != kNone && leaf == kNone && cur != kNone && (cur & kLeafBit)) {
+            if (slot != kNone && leaf == kNone && cur != kNone && (cur & kLeafBit)) {
                 leaf = cur;
                 cur = sp ? st.get(--sp) : kNone;
             }
             const bool in = slot != kNone && cur != kNone && !(cur & kLeafBit);
-            if (__ballot(kPostpone ? (in && leaf == kNone) : in) == 0) break;
+            if (__ballot(in && leaf == kNone) == 0) break;
             if (in) {
                 const float4 *nodes = (kSpheres && sphere_phase) ? sc.snodes : sc.mnodes;
                 node_step(nodes, o, inv, t_min, t_cur, best.t, cur, sp, st);
             }
         }
         // ---- leaves ----------------------------------------------------------
-        if (kPostpone) {
-            if (slot != kNone && leaf != kNone) {
-                if (kSpheres && sphere_phase)
-                    sphere_leaf(sc, leaf & ~kLeafBit, o, d, t_min, t_cur, best);
-                else
-                    tri_leaf(sc, leaf & ~kLeafBit, o, d, t_min, t_cur, best);
-                leaf = kNone;
-            }
-        } else if (slot != kNone && cur != kNone) {
+        if (slot != kNone && leaf != kNone) {
             if (kSpheres && sphere_phase)
-                sphere_leaf(sc, cur & ~kLeafBit, o, d, t_min, t_cur, best);
+                sphere_leaf(sc, leaf & ~kLeafBit, o, d, t_min, t_cur, best);
             else
-                tri_leaf(sc, cur & ~kLeafBit, o, d, t_min, t_cur, best);
-            cur = sp ? st.get(--sp) : kNone;
+                tri_leaf(sc, leaf & ~kLeafBit, o, d, t_min, t_cur, best);
+            leaf = kNone;
         }
         // ---- phase end: spheres -> mesh (world_hit, :471-479) or finish ----
         if (slot != kNone && cur == kNone && leaf == kNone) {

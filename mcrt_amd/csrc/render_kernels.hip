@@ -70,10 +70,10 @@ struct ExtIO {
     }
 };
 
-template <int kBlock, bool kSpheres, bool kPostpone>
+template <int kBlock, bool kSpheres>
 __global__ __launch_bounds__(kBlock) void k_extend(ExtArgs a) {
     __shared__ uint32_t lds[kShortStack * kBlock];
-    trace_persistent<kBlock, kSpheres, kPostpone>(a.scene, ExtIO{a}, lds, a.ovf, a.ovf_stride);
+    trace_persistent<kBlock, kSpheres>(a.scene, ExtIO{a}, lds, a.ovf, a.ovf_stride);
 }
 
 // ---------------------------------------------------------------------------
@@ -232,10 +232,10 @@ struct OccIO {
     }
 };
 
-template <int kBlock, bool kSpheres, bool kPostpone>
+template <int kBlock, bool kSpheres>
 __global__ __launch_bounds__(kBlock) void k_occlude(OccArgs a) {
     __shared__ uint32_t lds[kShortStack * kBlock];
-    trace_persistent<kBlock, kSpheres, kPostpone>(a.scene, OccIO{a}, lds, a.ovf, a.ovf_stride);
+    trace_persistent<kBlock, kSpheres>(a.scene, OccIO{a}, lds, a.ovf, a.ovf_stride);
 }
 
 // ---------------------------------------------------------------------------
@@ -278,10 +278,10 @@ struct TraceIO {
     }
 };
 
-template <int kBlock, bool kSpheres, bool kPostpone>
+template <int kBlock, bool kSpheres>
 __global__ __launch_bounds__(kBlock) void k_trace(TraceArgs a) {
     __shared__ uint32_t lds[kShortStack * kBlock];
-    trace_persistent<kBlock, kSpheres, kPostpone>(a.scene, TraceIO{a}, lds, a.ovf, a.ovf_stride);
+    trace_persistent<kBlock, kSpheres>(a.scene, TraceIO{a}, lds, a.ovf, a.ovf_stride);
 }
 
 __global__ void k_fill_u32(uint32_t *p, uint32_t n, uint32_t v) {
@@ -304,26 +304,23 @@ hipError_t launch_generate(const GenArgs &a, hipStream_t s) {
     return hipGetLastError();
 }
 
-// Traversal kernel variant for a scene: with or without the sphere phase,
-// with or without postponed leaves.
-template <int kBlock, bool kSpheres, bool kPostpone>
+// Traversal kernel variant for a scene: with or without the sphere phase.
+template <int kBlock, bool kSpheres>
 struct ExtendK {
-    static const void *fn() { return reinterpret_cast<const void *>(&k_extend<kBlock, kSpheres, kPostpone>); }
+    static const void *fn() { return reinterpret_cast<const void *>(&k_extend<kBlock, kSpheres>); }
 };
-template <int kBlock, bool kSpheres, bool kPostpone>
+template <int kBlock, bool kSpheres>
 struct TraceK {
-    static const void *fn() { return reinterpret_cast<const void *>(&k_trace<kBlock, kSpheres, kPostpone>); }
+    static const void *fn() { return reinterpret_cast<const void *>(&k_trace<kBlock, kSpheres>); }
 };
-template <int kBlock, bool kSpheres, bool kPostpone>
+template <int kBlock, bool kSpheres>
 struct OccludeK {
-    static const void *fn() { return reinterpret_cast<const void *>(&k_occlude<kBlock, kSpheres, kPostpone>); }
+    static const void *fn() { return reinterpret_cast<const void *>(&k_occlude<kBlock, kSpheres>); }
 };
 
-template <template <int, bool, bool> class K>
+template <template <int, bool> class K>
 const void *traversal_kernel(const DevScene &sc) {
-    const bool sph = sc.sroot != kNone, pp = sc.postpone_leaves != 0;
-    if (sph) return pp ? K<kTraceBlock, true, true>::fn() : K<kTraceBlock, true, false>::fn();
-    return pp ? K<kTraceBlock, false, true>::fn() : K<kTraceBlock, false, false>::fn();
+    return sc.sroot != kNone ? K<kTraceBlock, true>::fn() : K<kTraceBlock, false>::fn();
 }
 
 template <class Args>
