@@ -61,6 +61,7 @@ struct DevScene {
     const float4 *env;
     uint32_t env_w, env_h;
     uint32_t stack_depth;     // traversal stack entries needed per lane
+    uint32_t refill;          // idle lanes (1..64) at which a traversal wave loads new rays
     // Emitters for next-event estimation: emissive triangles and spheres,
     // chosen with probability proportional to area x luminance.
     //   emit[2e] = (prob, kind | leaf-order index bits, area, -),
@@ -195,9 +196,9 @@ __device__ __forceinline__ V3 tri_normal(const DevScene &sc, uint32_t ordered, f
 // the scheduling of rays onto lanes leave every result unchanged.
 //
 // Scheduling: a wave pulls chunks of kChunk ray slots with one atomic and
-// hands them out to idle lanes whenever at least kRefill lanes are idle, so
-// lanes whose rays finished early are refilled instead of idling until the
-// slowest ray of the wave ends.  Inside an iteration every lane first walks
+// hands them out to idle lanes whenever at least DevScene::refill lanes are
+// idle, so lanes whose rays finished early are refilled instead of idling
+// until the slowest ray of the wave ends.  Inside an iteration every lane first walks
 // internal nodes until it reaches a leaf (or finishes), then the leaves are
 // intersected together (the "while-while" order).
 //
@@ -208,7 +209,6 @@ __device__ __forceinline__ V3 tri_normal(const DevScene &sc, uint32_t ordered, f
 constexpr float kCullSlack = 1.0009765625f;  // 1 + 2^-10
 constexpr uint32_t kShortStack = 16;
 constexpr uint32_t kChunk = 256;
-constexpr uint32_t kRefill = 16;
 
 struct Best {
     float t;
@@ -403,7 +403,7 @@ __device__ __forceinline__ void trace_persistent(const DevScene &sc, const IO &i
         // ---- refill idle lanes --------------------------------------------
         const uint64_t idle = __ballot(slot == kNone);
         const uint32_t nidle = (uint32_t)__popcll(idle);
-        if (nidle >= kRefill) {
+        if (nidle >= sc.refill) {
             if (wbase >= wend && more) {
                 uint32_t g = 0;
                 if (lane == 0) g = atomicAdd(io.work(), kChunk);
