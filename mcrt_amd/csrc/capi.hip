@@ -388,7 +388,7 @@ extern "C" int mcrt_upload_scene(mcrt_ctx *c, const GpuSphere *spheres, uint32_t
             if (g.type_pad[1] != 0 && g.type_pad[0] != MCRT_MAT_EMISSIVE && m < materials->mat_count)
                 return fail(c, MCRT_ERR_BAD_ARG,
                             "material %u samples albedo texture layer %u; texture sampling is not "
-                            "implemented in this build",
+                            "implemented by this library",
                             m, g.type_pad[1]);
         }
     }

@@ -138,7 +138,7 @@ def _opt_u32(args, flag):
 def main(argv=None) -> int:
     args = list(sys.argv[1:] if argv is None else argv)
     if "--headless" not in args:
-        sys.stderr.write("error: only --headless mode is provided by this build\n")
+        sys.stderr.write("error: only --headless mode is provided (the interactive window is not)\n")
         return 1
     scene = get_next_after(args, "--headless")
     if scene is None:

@@ -17,7 +17,7 @@ Restates ``load_scene_from_str`` (src/scene.rs:640-978) and
   (src/scene.rs:902-907).
 
 All geometry arithmetic is FP32 in the reference's operation order.  The
-``mesh`` object type (OBJ files) is outside this build's scope and raises
+``mesh`` object type (OBJ files) is not supported here and raises
 :class:`SceneError`.
 """
 from __future__ import annotations
@@ -379,7 +379,7 @@ def load_scene_from_str(text: str, source: str = "<inline>",
         elif kind == "mesh":
             resolve(obj["material"], what)
             raise SceneError("MeshLoad", f"failed to load mesh '{obj['path']}': OBJ meshes are "
-                                         "outside the scope of this build")
+                                         "not supported by this loader")
         else:
             raise SceneError("Parse", f"{what}: unknown object type {kind!r}")
 

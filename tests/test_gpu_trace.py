@@ -3,7 +3,7 @@ CPU oracle.
 
 Bar: primitive ids bit-exact and hit t bit-exact (the kernel evaluates the
 shader's slab and Möller–Trumbore expressions with the same FP32 roundings,
-so the north-star tolerance of 1e-5 relative is met with zero error).
+so the stated tolerance of 1e-5 relative is met with zero error).
 """
 import os
 
