@@ -174,7 +174,10 @@ def test_bilateral_matches_oracle(ctx):
     w = configs.c1(width=128, height=96)
     ctx.upload_scene(w.buffers)
     acc, gb = ctx.render(w.camera, 128, 96, 4, 0, 8)
-    for radius, ss, sd, npow in ((3, 2.0, 0.5, 8.0), (1, 1.0, 0.25, 2.0), (5, 3.0, 1.0, 16.0), (0, 2.0, 0.5, 8.0)):
+    # Whole exponents take the kernel's repeated-squaring path (0 included:
+    # pow(x, 0) = 1 even at x = 0); 7.5 and 0.3 take the general pow.
+    for radius, ss, sd, npow in ((3, 2.0, 0.5, 8.0), (1, 1.0, 0.25, 2.0), (5, 3.0, 1.0, 16.0), (0, 2.0, 0.5, 8.0),
+                                 (2, 1.5, 0.75, 7.5), (3, 2.0, 0.5, 0.3), (3, 2.0, 0.5, 0.0)):
         p = T.default_denoise_params()
         p["radius"], p["sigma_s"], p["sigma_d"], p["normal_pow"] = radius, ss, sd, npow
         g = ctx.denoise_bilateral(acc, gb, p)
