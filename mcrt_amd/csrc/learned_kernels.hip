@@ -93,107 +93,124 @@ __global__ __launch_bounds__(256) void k_pack(PackArgs a) {
 // at one horizontal tap kx, 32 channels) is read from LDS once and feeds
 // every output row r it contributes to (vertical tap ky = input row - r).
 // kFinal: the last layer, whose 3 outputs need only the first half.
-constexpr int kConvGrid = 1024;  // 256 CUs x 4 resident blocks
-
+// Block layout: the 16 x 16 output tile is split into 4 groups of 4 rows;
+// a wave computes one row group for one 16-channel half of the outputs
+// (8 waves per block), or for the 3 output channels of the last layer
+// (4 waves).  Holding one half of the weights per wave (36 VGPRs of B
+// fragments instead of 72) doubles the resident waves, which hide the
+// tile loads of the other blocks.
 template <bool kFinal>
-__global__ __launch_bounds__(256) void k_conv3x3(ConvArgs a) {
-    __shared__ bf16 s_in[kH * kH * kPS];
+__global__ __launch_bounds__(kFinal ? 256 : 512) void k_conv3x3(ConvArgs a) {
+    constexpr int kThreads = kFinal ? 256 : 512;
+    __shared__ __align__(16) bf16 s_in[kH * kH * kPS];
     const int tid = threadIdx.x;
     const int wave = tid >> 6, lane = tid & 63;
     const int m = lane & 15, kq = lane >> 4;
-    constexpr int kHalves = kFinal ? 1 : 2;
-    bf16x8 wfrag[9][kHalves];
+    const int rg = wave & 3;                     // row group: output rows 4*rg .. 4*rg+3
+    const int half = kFinal ? 0 : (wave >> 2);   // output channels 16*half .. 16*half+15
+    bf16x8 wfrag[9];
 #pragma unroll
     for (int tap = 0; tap < 9; ++tap)
-#pragma unroll
-        for (int h = 0; h < kHalves; ++h)
-            wfrag[tap][h] = *reinterpret_cast<const bf16x8 *>(a.w + (tap * kC + 16 * h + m) * kC + kq * 8);
-    float bias[kHalves];
-#pragma unroll
-    for (int h = 0; h < kHalves; ++h) bias[h] = a.bias ? a.bias[16 * h + m] : 0.0f;
+        wfrag[tap] = *reinterpret_cast<const bf16x8 *>(a.w + (tap * kC + 16 * half + m) * kC + kq * 8);
+    const float bias = a.bias ? a.bias[16 * half + m] : 0.0f;
 
     const int tiles_x = (a.W + kT - 1) / kT, tiles_y = (a.H + kT - 1) / kT;
     const int per_image = tiles_x * tiles_y, n_tiles = per_image * a.B;
+    // Software pipeline: each thread's share of a tile's input (with halo,
+    // zero outside the image) is loaded into registers one tile ahead, so
+    // the loads are in flight while the current tile computes.
+    constexpr int kLoads = (kH * kH * 4 + kThreads - 1) / kThreads;
+    uint4 pre[kLoads], pmask[kLoads];
+    auto fetch = [&](int t) {
+        const int b = t / per_image, rem = t - b * per_image;
+        const int y0 = (rem / tiles_x) * kT, x0 = (rem % tiles_x) * kT;
+#pragma unroll
+        for (int k = 0; k < kLoads; ++k) {
+            const int i = tid + k * kThreads;
+            const int pix = i >> 2, q = i & 3;
+            const int hy = pix / kH, hx = pix - hy * kH;
+            const int gy = y0 + hy - 1, gx = x0 + hx - 1;
+            pre[k] = make_uint4(0, 0, 0, 0);
+            pmask[k] = make_uint4(0, 0, 0, 0);
+            if (i < kH * kH * 4 && gy >= 0 && gy < a.H && gx >= 0 && gx < a.W) {
+                const size_t off = (((size_t)b * a.H + gy) * a.W + gx) * kC + q * 8;
+                pre[k] = *reinterpret_cast<const uint4 *>(a.in + off);
+                if (a.mask) pmask[k] = *reinterpret_cast<const uint4 *>(a.mask + off);
+            }
+        }
+    };
+    if ((int)blockIdx.x < n_tiles) fetch(blockIdx.x);
     for (int tile = blockIdx.x; tile < n_tiles; tile += gridDim.x) {
         const int b = tile / per_image, rem = tile - b * per_image;
         const int y0 = (rem / tiles_x) * kT, x0 = (rem % tiles_x) * kT;
         __syncthreads();  // the previous tile is done with s_in
-        // Input tile with halo; zero outside the image.
-        for (int i = tid; i < kH * kH * 4; i += 256) {
-            const int pix = i >> 2, q = i & 3;
-            const int hy = pix / kH, hx = pix - hy * kH;
-            const int gy = y0 + hy - 1, gx = x0 + hx - 1;
-            uint4 v = make_uint4(0, 0, 0, 0);
-            if (gy >= 0 && gy < a.H && gx >= 0 && gx < a.W) {
-                const size_t off = (((size_t)b * a.H + gy) * a.W + gx) * kC + q * 8;
-                v = *reinterpret_cast<const uint4 *>(a.in + off);
-                if (a.mask) v = apply_mask(v, *reinterpret_cast<const uint4 *>(a.mask + off));
+#pragma unroll
+        for (int k = 0; k < kLoads; ++k) {
+            const int i = tid + k * kThreads;
+            if (i < kH * kH * 4) {
+                const int pix = i >> 2, q = i & 3;
+                *reinterpret_cast<uint4 *>(&s_in[pix * kPS + q * 8]) = a.mask ? apply_mask(pre[k], pmask[k]) : pre[k];
             }
-            *reinterpret_cast<uint4 *>(&s_in[pix * kPS + q * 8]) = v;
         }
         __syncthreads();
-        floatx4 acc[4][kHalves];
+        if (tile + (int)gridDim.x < n_tiles) fetch(tile + gridDim.x);
+        floatx4 acc[4];
 #pragma unroll
-        for (int r = 0; r < 4; ++r)
-#pragma unroll
-            for (int h = 0; h < kHalves; ++h) acc[r][h] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+        for (int r = 0; r < 4; ++r) acc[r] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
         for (int ir = 0; ir < 6; ++ir) {
 #pragma unroll
             for (int kx = 0; kx < 3; ++kx) {
                 const bf16x8 av =
-                    *reinterpret_cast<const bf16x8 *>(&s_in[((wave * 4 + ir) * kH + (m + kx)) * kPS + kq * 8]);
+                    *reinterpret_cast<const bf16x8 *>(&s_in[((rg * 4 + ir) * kH + (m + kx)) * kPS + kq * 8]);
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
                     const int ky = ir - r;
                     if (ky < 0 || ky > 2) continue;
-#pragma unroll
-                    for (int h = 0; h < kHalves; ++h)
-                        acc[r][h] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, wfrag[ky * 3 + kx][h], acc[r][h],
-                                                                             0, 0, 0);
+                    acc[r] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, wfrag[ky * 3 + kx], acc[r], 0, 0, 0);
                 }
             }
         }
         // D layout: lane holds rows 4*kq + j (pixel x), column m (output channel).
         if (kFinal) {
-            // out_log = log1p(max(noisy, 0)) + conv + bias, channels 0..2.
+            // out_log = log1p(max(noisy, 0)) + conv + bias, channels 0..2:
+            // the conv terms go through LDS so that each thread then reads
+            // and writes one pixel's float4.
+            float *s_f = reinterpret_cast<float *>(s_in);  // [pixel][3]
+            __syncthreads();  // every wave is done reading the input tile
+            if (m < 3) {
 #pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int gy = y0 + wave * 4 + r;
+                for (int r = 0; r < 4; ++r)
 #pragma unroll
-                for (int j = 0; j < 4; ++j) {
-                    const int gx = x0 + 4 * kq + j;
-                    if (gy < a.H && gx < a.W && m < 4) {
-                        const size_t p = ((size_t)b * a.H + gy) * a.W + gx;
-                        float *o = reinterpret_cast<float *>(a.out_log + p);
-                        if (m < 3) {
-                            const float nz = reinterpret_cast<const float *>(a.noisy + p)[m];
-                            o[m] = log1pf(fmaxf(nz, 0.0f)) + (acc[r][0][j] + bias[0]);
-                        } else {
-                            o[3] = 0.0f;
-                        }
-                    }
+                    for (int j = 0; j < 4; ++j) s_f[((rg * 4 + r) * kT + 4 * kq + j) * 3 + m] = acc[r][j] + bias;
+            }
+            __syncthreads();
+            for (int px = tid; px < kT * kT; px += kThreads) {
+                const int gy = y0 + (px >> 4), gx = x0 + (px & 15);
+                if (gy < a.H && gx < a.W) {
+                    const size_t p = ((size_t)b * a.H + gy) * a.W + gx;
+                    const float4 nz = a.noisy[p];
+                    a.out_log[p] = make_float4(log1pf(fmaxf(nz.x, 0.0f)) + s_f[px * 3 + 0],
+                                               log1pf(fmaxf(nz.y, 0.0f)) + s_f[px * 3 + 1],
+                                               log1pf(fmaxf(nz.z, 0.0f)) + s_f[px * 3 + 2], 0.0f);
                 }
             }
             continue;
         }
         __syncthreads();  // s_in is reused as the output tile
+        const int co = 16 * half + m;
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-            const int y = wave * 4 + r;
+            const int y = rg * 4 + r;
 #pragma unroll
-            for (int h = 0; h < kHalves; ++h) {
-                const int co = 16 * h + m;
-#pragma unroll
-                for (int j = 0; j < 4; ++j) {
-                    float v = acc[r][h][j] + bias[h];
-                    if (a.mode == 0) v = fmaxf(v, 0.0f);
-                    s_in[(y * kT + 4 * kq + j) * kPS + co] = f2b(v);
-                }
+            for (int j = 0; j < 4; ++j) {
+                float v = acc[r][j] + bias;
+                if (a.mode == 0) v = fmaxf(v, 0.0f);
+                s_in[(y * kT + 4 * kq + j) * kPS + co] = f2b(v);
             }
         }
         __syncthreads();
-        for (int i = tid; i < kT * kT * 4; i += 256) {
+        for (int i = tid; i < kT * kT * 4; i += kThreads) {
             const int pix = i >> 2, q = i & 3;
             const int y = pix >> 4, x = pix & 15;
             const int gy = y0 + y, gx = x0 + x;
@@ -422,13 +439,30 @@ hipError_t launch_pack(const PackArgs &a, hipStream_t s) {
     return hipGetLastError();
 }
 
+// Grid-stride launch sized to the blocks that are resident at once.
+template <bool kFinal>
+uint32_t conv_grid(size_t tiles) {
+    static uint32_t resident = 0;
+    if (!resident) {
+        int dev = 0, cus = 0, per_cu = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void *>(&k_conv3x3<kFinal>),
+                                                         kFinal ? 256 : 512, 0) != hipSuccess ||
+            cus < 1 || per_cu < 1)
+            resident = 1024;
+        else
+            resident = (uint32_t)(cus * per_cu);
+    }
+    return (uint32_t)std::min<size_t>(tiles, resident);
+}
+
 hipError_t launch_conv3x3(const ConvArgs &a, hipStream_t s) {
     const size_t tiles = (size_t)((a.W + kT - 1) / kT) * ((a.H + kT - 1) / kT) * a.B;
-    const dim3 grid((uint32_t)std::min<size_t>(tiles, kConvGrid));
     if (a.mode == 2)
-        hipLaunchKernelGGL(k_conv3x3<true>, grid, dim3(256), 0, s, a);
+        hipLaunchKernelGGL(k_conv3x3<true>, dim3(conv_grid<true>(tiles)), dim3(256), 0, s, a);
     else
-        hipLaunchKernelGGL(k_conv3x3<false>, grid, dim3(256), 0, s, a);
+        hipLaunchKernelGGL(k_conv3x3<false>, dim3(conv_grid<false>(tiles)), dim3(512), 0, s, a);
     return hipGetLastError();
 }
 
