@@ -94,7 +94,7 @@ __device__ __forceinline__ float safe_div(float x, float y) { return y != 0.0f ?
 
 template <int kBlock, bool kSpheres, bool kAdjoint>
 __global__ __launch_bounds__(kBlock) void k_diff(DiffArgs a) {
-    __shared__ uint32_t lds[kShortStack * kBlock];
+    __shared__ uint32_t lds[Stack::kShort * Stack::kWords * kBlock];
     __shared__ float red[kAdjoint ? kMaxParams * 4 * kBlock : 1];
     const uint32_t pix = blockIdx.x * kBlock + threadIdx.x;
     const uint32_t wh = a.width * a.height;

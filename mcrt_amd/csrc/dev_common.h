@@ -62,6 +62,7 @@ struct DevScene {
     uint32_t env_w, env_h;
     uint32_t stack_depth;     // traversal stack entries needed per lane
     uint32_t refill;          // idle lanes (1..64) at which a traversal wave loads new rays
+    uint32_t stack_cull;      // traversal kernels with the culling stack (TStack<true>)
     // Emitters for next-event estimation: emissive triangles and spheres,
     // chosen with probability proportional to area x luminance.
     //   emit[2e] = (prob, kind | leaf-order index bits, area, -),
@@ -202,12 +203,11 @@ __device__ __forceinline__ V3 tri_normal(const DevScene &sc, uint32_t ordered, f
 // internal nodes until it reaches a leaf (or finishes), then the leaves are
 // intersected together (the "while-while" order).
 //
-// Stack: the first kShortStack entries of each lane live in LDS
+// Stack: the first kShort entries of each lane live in LDS
 // (lds[i * kBlock + tid]); deeper entries go to a per-lane column of a
-// global overflow buffer (ovf[(i - kShortStack) * ovf_stride + gtid]).
+// global overflow buffer (ovf[(i - kShort) * words * ovf_stride + gtid]).
 // ---------------------------------------------------------------------------
 constexpr float kCullSlack = 1.0009765625f;  // 1 + 2^-10
-constexpr uint32_t kShortStack = 16;
 constexpr uint32_t kChunk = 256;
 
 struct Best {
@@ -221,25 +221,56 @@ __device__ __forceinline__ bool better(float t, uint32_t id, const Best &b) {
     return t < b.t || (t == b.t && id < b.id);
 }
 
-struct Stack {
+// Traversal stack of one lane.  kCull = false: entries are node references,
+// 16 in LDS.  kCull = true: an entry also holds the entry distance of its box,
+// measured when it was pushed, and a pop skips entries that start beyond the
+// lane's current cull limit without fetching them (the rule that culls a
+// child at its parent, applied with the closer hit found since); 10 entries
+// in LDS keep the block's LDS at the plain stack's occupancy.
+template <bool kCull>
+struct TStack {
+    static constexpr uint32_t kShort = kCull ? 10u : 16u;
+    static constexpr uint32_t kWords = kCull ? 2u : 1u;  // 32-bit words per entry
     uint32_t *lds;
     uint32_t *ovf;
     uint32_t ovf_stride, tid, gtid, block;
-    __device__ __forceinline__ void set(uint32_t i, uint32_t v) const {
-        if (i < kShortStack)
+    __device__ __forceinline__ void set(uint32_t i, uint32_t v, float te) const {
+        if (i < kShort) {
             lds[i * block + tid] = v;
-        else
-            ovf[(size_t)(i - kShortStack) * ovf_stride + gtid] = v;
+            if (kCull) lds[(kShort + i) * block + tid] = __float_as_uint(te);
+        } else {
+            const size_t e = (size_t)(i - kShort) * kWords * ovf_stride + gtid;
+            ovf[e] = v;
+            if (kCull) ovf[e + ovf_stride] = __float_as_uint(te);
+        }
     }
-    // The two reads go through explicitly LDS- and global-qualified pointers.
+    // The reads go through explicitly LDS- and global-qualified pointers.
     // Through plain pointers the compiler merges them into one flat load,
     // which routes every LDS pop through the vector-memory path and makes each
     // node step wait on both counters.
-    __device__ __forceinline__ uint32_t get(uint32_t i) const {
-        if (i < kShortStack) return ((const __attribute__((address_space(3))) uint32_t *)lds)[i * block + tid];
-        return ((const __attribute__((address_space(1))) uint32_t *)ovf)[(size_t)(i - kShortStack) * ovf_stride + gtid];
+    __device__ __forceinline__ uint32_t get(uint32_t i, float &te) const {
+        if (i < kShort) {
+            const __attribute__((address_space(3))) uint32_t *l = (const __attribute__((address_space(3))) uint32_t *)lds;
+            if (kCull) te = __uint_as_float(l[(kShort + i) * block + tid]);
+            return l[i * block + tid];
+        }
+        const __attribute__((address_space(1))) uint32_t *g = (const __attribute__((address_space(1))) uint32_t *)ovf;
+        const size_t e = (size_t)(i - kShort) * kWords * ovf_stride + gtid;
+        if (kCull) te = __uint_as_float(g[e + ovf_stride]);
+        return g[e];
+    }
+    // Top entry whose box may still hold a hit (entry distance within lim),
+    // or kNone when the stack runs out.
+    __device__ __forceinline__ uint32_t pop(uint32_t &sp, float lim) const {
+        while (sp) {
+            float te = 0.0f;
+            const uint32_t v = get(--sp, te);
+            if (!kCull || !(te > lim)) return v;
+        }
+        return kNone;
     }
 };
+using Stack = TStack<false>;
 
 // Root entry of one BVH for a ray: the root box test of the reference
 // (the root node is popped and slab-tested first, :375-379).
@@ -255,9 +286,10 @@ __device__ __forceinline__ uint32_t enter_root(const DevScene &sc, bool sphere, 
 }
 
 // One internal-node step: slab-test both children, descend near-first.
+template <class S>
 __device__ __forceinline__ void node_step(const float4 *nodes, V3 o, V3 inv, float t_min,
                                           float t_max, float best_t, uint32_t &cur, uint32_t &sp,
-                                          const Stack &st) {
+                                          const S &st) {
     const float4 *n = nodes + 4u * cur;
     float4 q0 = n[0], q1 = n[1], q2 = n[2], q3 = n[3];
     float te0, tx0, te1, tx1;
@@ -273,7 +305,7 @@ __device__ __forceinline__ void node_step(const float4 *nodes, V3 o, V3 inv, flo
             nearc = c1;
             farc = c0;
         }
-        st.set(sp, farc);
+        st.set(sp, farc, te1 < te0 ? te0 : te1);
         ++sp;
         cur = nearc;
     } else if (h0) {
@@ -281,7 +313,7 @@ __device__ __forceinline__ void node_step(const float4 *nodes, V3 o, V3 inv, flo
     } else if (h1) {
         cur = c1;
     } else {
-        cur = sp ? st.get(--sp) : kNone;
+        cur = st.pop(sp, lim);
     }
 }
 
@@ -355,7 +387,7 @@ __device__ __forceinline__ Best trace_one(const DevScene &sc, V3 o, V3 d, float 
                 node_step(sc.snodes, o, inv, t_min, t_max, sb.t, cur, sp, st);
             } else {
                 sphere_leaf(sc, cur & ~kLeafBit, o, d, t_min, t_max, sb);
-                cur = sp ? st.get(--sp) : kNone;
+                cur = st.pop(sp, sb.t * kCullSlack);
             }
         }
     }
@@ -370,7 +402,7 @@ __device__ __forceinline__ Best trace_one(const DevScene &sc, V3 o, V3 d, float 
             node_step(sc.mnodes, o, inv, t_min, t_mesh, mb.t, cur, sp, st);
         } else {
             tri_leaf(sc, cur & ~kLeafBit, o, d, t_min, t_mesh, mb);
-            cur = sp ? st.get(--sp) : kNone;
+            cur = st.pop(sp, mb.t * kCullSlack);
         }
     }
     return mb.prim != kNone ? mb : sb;
@@ -378,12 +410,13 @@ __device__ __forceinline__ Best trace_one(const DevScene &sc, V3 o, V3 d, float 
 
 // IO policy: count(), work(), load(slot, o, d, t_min, t_max), store(slot, best).
 // kSpheres = false compiles the variant for scenes without spheres (no
-// sphere phase state), launched when the sphere BVH is empty.
-template <int kBlock, bool kSpheres, class IO>
+// sphere phase state), launched when the sphere BVH is empty; kCull selects
+// the culling stack (lds holds TStack<kCull>::kShort * kWords words per lane).
+template <int kBlock, bool kSpheres, bool kCull, class IO>
 __device__ __forceinline__ void trace_persistent(const DevScene &sc, const IO &io, uint32_t *lds,
                                                  uint32_t *ovf, uint32_t ovf_stride) {
     const uint32_t lane = __lane_id();
-    Stack st{lds, ovf, ovf_stride, threadIdx.x, blockIdx.x * kBlock + threadIdx.x, (uint32_t)kBlock};
+    TStack<kCull> st{lds, ovf, ovf_stride, threadIdx.x, blockIdx.x * kBlock + threadIdx.x, (uint32_t)kBlock};
     const uint32_t total = io.count();
     uint32_t wbase = 0, wend = 0;  // wave-local slot range
     bool more = true;
@@ -446,7 +479,7 @@ __device__ __forceinline__ void trace_persistent(const DevScene &sc, const IO &i
         for (;;) {
             if (slot != kNone && leaf == kNone && cur != kNone && (cur & kLeafBit)) {
                 leaf = cur;
-                cur = sp ? st.get(--sp) : kNone;
+                cur = st.pop(sp, best.t * kCullSlack);
             }
             const bool in = slot != kNone && cur != kNone && !(cur & kLeafBit);
             if (__ballot(in && leaf == kNone) == 0) break;

@@ -175,6 +175,8 @@ hipError_t launch_bilateral(const BilateralArgs &a, hipStream_t s);
 // which: 0 extend, 1 shade, 2 trace, 3 occlude (the traversal variant the scene selects)
 hipError_t occupancy(int which, const DevScene &sc, int *blocks_per_cu);
 int trace_block();
-uint32_t short_stack();
+// LDS entries and 32-bit words per entry of the plain (cull = false) or
+// culling traversal stack, for sizing the overflow columns.
+void stack_layout(bool cull, uint32_t *short_entries, uint32_t *words_per_entry);
 
 }  // namespace mcrt

@@ -70,10 +70,10 @@ struct ExtIO {
     }
 };
 
-template <int kBlock, bool kSpheres>
+template <int kBlock, bool kSpheres, bool kCull>
 __global__ __launch_bounds__(kBlock) void k_extend(ExtArgs a) {
-    __shared__ uint32_t lds[kShortStack * kBlock];
-    trace_persistent<kBlock, kSpheres>(a.scene, ExtIO{a}, lds, a.ovf, a.ovf_stride);
+    __shared__ uint32_t lds[TStack<kCull>::kShort * TStack<kCull>::kWords * kBlock];
+    trace_persistent<kBlock, kSpheres, kCull>(a.scene, ExtIO{a}, lds, a.ovf, a.ovf_stride);
 }
 
 // ---------------------------------------------------------------------------
@@ -232,10 +232,10 @@ struct OccIO {
     }
 };
 
-template <int kBlock, bool kSpheres>
+template <int kBlock, bool kSpheres, bool kCull>
 __global__ __launch_bounds__(kBlock) void k_occlude(OccArgs a) {
-    __shared__ uint32_t lds[kShortStack * kBlock];
-    trace_persistent<kBlock, kSpheres>(a.scene, OccIO{a}, lds, a.ovf, a.ovf_stride);
+    __shared__ uint32_t lds[TStack<kCull>::kShort * TStack<kCull>::kWords * kBlock];
+    trace_persistent<kBlock, kSpheres, kCull>(a.scene, OccIO{a}, lds, a.ovf, a.ovf_stride);
 }
 
 // ---------------------------------------------------------------------------
@@ -278,10 +278,10 @@ struct TraceIO {
     }
 };
 
-template <int kBlock, bool kSpheres>
+template <int kBlock, bool kSpheres, bool kCull>
 __global__ __launch_bounds__(kBlock) void k_trace(TraceArgs a) {
-    __shared__ uint32_t lds[kShortStack * kBlock];
-    trace_persistent<kBlock, kSpheres>(a.scene, TraceIO{a}, lds, a.ovf, a.ovf_stride);
+    __shared__ uint32_t lds[TStack<kCull>::kShort * TStack<kCull>::kWords * kBlock];
+    trace_persistent<kBlock, kSpheres, kCull>(a.scene, TraceIO{a}, lds, a.ovf, a.ovf_stride);
 }
 
 __global__ void k_fill_u32(uint32_t *p, uint32_t n, uint32_t v) {
@@ -304,23 +304,26 @@ hipError_t launch_generate(const GenArgs &a, hipStream_t s) {
     return hipGetLastError();
 }
 
-// Traversal kernel variant for a scene: with or without the sphere phase.
-template <int kBlock, bool kSpheres>
+// Traversal kernel variant for a scene: with or without the sphere phase,
+// with the plain or the culling stack.
+template <int kBlock, bool kSpheres, bool kCull>
 struct ExtendK {
-    static const void *fn() { return reinterpret_cast<const void *>(&k_extend<kBlock, kSpheres>); }
+    static const void *fn() { return reinterpret_cast<const void *>(&k_extend<kBlock, kSpheres, kCull>); }
 };
-template <int kBlock, bool kSpheres>
+template <int kBlock, bool kSpheres, bool kCull>
 struct TraceK {
-    static const void *fn() { return reinterpret_cast<const void *>(&k_trace<kBlock, kSpheres>); }
+    static const void *fn() { return reinterpret_cast<const void *>(&k_trace<kBlock, kSpheres, kCull>); }
 };
-template <int kBlock, bool kSpheres>
+template <int kBlock, bool kSpheres, bool kCull>
 struct OccludeK {
-    static const void *fn() { return reinterpret_cast<const void *>(&k_occlude<kBlock, kSpheres>); }
+    static const void *fn() { return reinterpret_cast<const void *>(&k_occlude<kBlock, kSpheres, kCull>); }
 };
 
-template <template <int, bool> class K>
+template <template <int, bool, bool> class K>
 const void *traversal_kernel(const DevScene &sc) {
-    return sc.sroot != kNone ? K<kTraceBlock, true>::fn() : K<kTraceBlock, false>::fn();
+    const bool sph = sc.sroot != kNone;
+    if (sc.stack_cull) return sph ? K<kTraceBlock, true, true>::fn() : K<kTraceBlock, false, true>::fn();
+    return sph ? K<kTraceBlock, true, false>::fn() : K<kTraceBlock, false, false>::fn();
 }
 
 template <class Args>
@@ -366,7 +369,10 @@ hipError_t occupancy(int which, const DevScene &sc, int *blocks_per_cu) {
     return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, f, which == 1 ? 256 : kTraceBlock, 0);
 }
 
-uint32_t short_stack() { return kShortStack; }
+void stack_layout(bool cull, uint32_t *short_entries, uint32_t *words_per_entry) {
+    *short_entries = cull ? TStack<true>::kShort : TStack<false>::kShort;
+    *words_per_entry = cull ? TStack<true>::kWords : TStack<false>::kWords;
+}
 
 int trace_block() { return kTraceBlock; }
 
