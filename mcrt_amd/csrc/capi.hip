@@ -49,7 +49,9 @@ struct mcrt_ctx {
     bool has_scene = false;
     DevBuf mnodes, mtris, tshade, snodes, sph, mats, env;
     DevBuf emit, emit_cdf, tri_emit, sph_emit;  // next-event estimation emitters
-    DevBuf ovf;  // traversal-stack overflow for trees deeper than the LDS short stack
+    DevBuf ovf, ovf2;  // traversal-stack overflow for trees deeper than the LDS short stack (per stream)
+    hipStream_t side_stream = nullptr;  // second stream of a split batch (default priority)
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     DevScene scene{};
     uint32_t n_tris = 0, n_spheres = 0;
     bool nee = false;  // next-event estimation (mcrt_set_nee)
@@ -66,7 +68,7 @@ struct mcrt_ctx {
     int spatial_radius = -1;
     float spatial_sigma = 0.0f;
     // last render bookkeeping
-    uint32_t last_batches = 0, last_depth = 0;
+    uint32_t last_batches = 0, last_depth = 0, last_halves = 1;
     uint64_t last_samples = 0;
     // profiling
     bool prof = false;
@@ -297,15 +299,16 @@ int grid_for(mcrt_ctx *c, int which, uint32_t *grid) {
 // Overflow columns for traversal stacks deeper than the LDS short stack:
 // (depth - short) entries of the stack layout's words (node reference, and
 // the entry distance for the culling stack) for every thread of a grid.
-int ensure_overflow(mcrt_ctx *c, uint32_t grid_threads, bool cull, uint32_t **ptr) {
+int ensure_overflow(mcrt_ctx *c, uint32_t grid_threads, bool cull, uint32_t which, uint32_t **ptr) {
     *ptr = nullptr;
     uint32_t sh = 0, words = 0;
     stack_layout(cull, &sh, &words);
     const uint32_t depth = c->scene.stack_depth;
     if (depth <= sh) return MCRT_OK;
-    int rc = ensure(c, c->ovf, (size_t)(depth - sh) * words * grid_threads * sizeof(uint32_t));
+    DevBuf &buf = which ? c->ovf2 : c->ovf;
+    int rc = ensure(c, buf, (size_t)(depth - sh) * words * grid_threads * sizeof(uint32_t));
     if (rc) return rc;
-    *ptr = (uint32_t *)c->ovf.p;
+    *ptr = (uint32_t *)buf.p;
     return MCRT_OK;
 }
 
@@ -344,7 +347,8 @@ extern "C" void mcrt_destroy(mcrt_ctx *c) {
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->last_stream);
     (void)hipStreamSynchronize(c->own_stream);
-    for (DevBuf *b : {&c->mnodes, &c->mtris, &c->tshade, &c->ovf, &c->snodes, &c->sph, &c->mats,
+    if (c->side_stream) (void)hipStreamSynchronize(c->side_stream);
+    for (DevBuf *b : {&c->mnodes, &c->mtris, &c->tshade, &c->ovf, &c->ovf2, &c->snodes, &c->sph, &c->mats,
                       &c->env, &c->A[0], &c->A[1], &c->B[0], &c->B[1], &c->C[0], &c->C[1], &c->H,
                       &c->samples, &c->counters, &c->host_accum, &c->host_gbuf, &c->host_out,
                       &c->host_accum_in, &c->trace_o, &c->trace_d, &c->trace_id, &c->trace_t,
@@ -358,6 +362,9 @@ extern "C" void mcrt_destroy(mcrt_ctx *c) {
     }
     for (hipEvent_t e : c->pool) (void)hipEventDestroy(e);
     if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
+    if (c->side_stream) (void)hipStreamDestroy(c->side_stream);
+    if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
+    if (c->ev_join) (void)hipEventDestroy(c->ev_join);
     delete c;
 }
 
@@ -619,8 +626,16 @@ extern "C" int mcrt_render_aux_dev(mcrt_ctx *c, const CameraUniform *cam, uint32
         c->cap_nee = cap;
     }
     const uint32_t n_batches = (spp + F - 1) / F;
+    // Cache-resident scenes trace a batch of at least 4 frames as two frame
+    // halves on two streams: a bounce's persistent launch on one half drains
+    // its slowest rays while the other half's launches fill the device (the
+    // C2 frame at 1080p: 6 spp 5% faster, 28 spp 2%).  Scenes that spill the
+    // L2, one-frame halves (split by pixels) and renders with next-event
+    // estimation measured slower, and stay on one stream.
+    const bool split = c->scene.stack_cull == 0 && !nee && F >= 4;
+    const uint32_t halves = split ? 2u : 1u;
     const uint32_t per_batch = counters_per_batch(max_depth);
-    const size_t n_counters = (size_t)per_batch * n_batches;
+    const size_t n_counters = (size_t)per_batch * n_batches * halves;
     if ((rc = ensure(c, c->counters, n_counters * sizeof(uint32_t)))) return rc;
     uint32_t *counters = (uint32_t *)c->counters.p;
     HIP_TRY(c, launch_fill_u32(counters, (uint32_t)n_counters, 0u, s));
@@ -629,97 +644,137 @@ extern "C" int mcrt_render_aux_dev(mcrt_ctx *c, const CameraUniform *cam, uint32
     grid_for(c, 0, &g_ext);
     if (nee) grid_for(c, 3, &g_occ);
     const uint32_t ovf_threads = std::max(g_ext, g_occ) * (uint32_t)trace_block();
-    uint32_t *ovf = nullptr;
-    if ((rc = ensure_overflow(c, ovf_threads, c->scene.stack_cull != 0, &ovf))) return rc;
+    uint32_t *ovf[2] = {nullptr, nullptr};
+    for (uint32_t h = 0; h < halves; ++h)
+        if ((rc = ensure_overflow(c, ovf_threads, c->scene.stack_cull != 0, h, &ovf[h]))) return rc;
+    hipStream_t hs[2] = {s, s};
+    if (split) {
+        if (!c->side_stream) {
+            HIP_TRY(c, hipStreamCreateWithFlags(&c->side_stream, hipStreamNonBlocking));
+            HIP_TRY(c, hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
+            HIP_TRY(c, hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
+        }
+        hs[1] = c->side_stream;
+    }
 
     const uint32_t last_frame = first_frame + spp - 1u;
     for (uint32_t bi = 0; bi < n_batches; ++bi) {
         const uint32_t f0 = first_frame + bi * F;
         const uint32_t nf = std::min<uint32_t>(F, spp - bi * F);
-        const uint32_t np = wh * nf;
-        uint32_t *cnt = counters + (size_t)bi * per_batch;  // [max_depth+1] path counts
-        uint32_t *wext = cnt + (max_depth + 1u);
-        uint32_t *wsh = wext + (max_depth + 1u);
-        uint32_t *scnt = wsh + (max_depth + 1u);  // [max_depth] shadow rays per bounce
-        uint32_t *wocc = scnt + max_depth;
-        HIP_TRY(c, launch_fill_u32(cnt, 1u, np, s));
-        GenArgs g{};
-        g.cam = *cam;
-        g.width = width;
-        g.height = height;
-        g.wh = wh;
-        g.frame0 = f0;
-        g.n_paths = np;
-        g.A = (float4 *)c->A[0].p;
-        g.B = (float4 *)c->B[0].p;
-        g.C = (float4 *)c->C[0].p;
-        g.samples = nee ? (float4 *)c->samples.p : nullptr;
-        g.P = nee ? (float *)c->P[0].p : nullptr;
-        {
-            Timed t(c, "generate", s);
-            HIP_TRY(c, launch_generate(g, s));
+        const bool two = split && nf >= 4;
+        const uint32_t nh = two ? 2u : 1u;
+        // Half h: frames [fo[h], fo[h] + fn[h]) of all pixels; its queue slots
+        // start at base[h] = wh * fo[h].
+        uint32_t fo[2] = {0u, 0u}, fn[2] = {nf, nf};
+        const uint32_t po[2] = {0u, 0u}, pn[2] = {wh, wh};
+        size_t base[2] = {0u, 0u};
+        if (two) {
+            fn[0] = nf / 2u;
+            fo[1] = fn[0];
+            fn[1] = nf - fn[0];
+            base[1] = (size_t)wh * fo[1];
+        }
+        const bool has_last = last_frame >= f0 && last_frame < f0 + nf;
+        if (two) {
+            HIP_TRY(c, hipEventRecord(c->ev_fork, s));
+            HIP_TRY(c, hipStreamWaitEvent(hs[1], c->ev_fork, 0));
+        }
+        uint32_t *cnt[2], *wext[2], *wsh[2], *scnt[2], *wocc[2];
+        for (uint32_t h = 0; h < nh; ++h) {
+            const uint32_t np = pn[h] * fn[h];
+            cnt[h] = counters + ((size_t)bi * halves + h) * per_batch;  // [max_depth+1] path counts
+            wext[h] = cnt[h] + (max_depth + 1u);
+            wsh[h] = wext[h] + (max_depth + 1u);
+            scnt[h] = wsh[h] + (max_depth + 1u);  // [max_depth] shadow rays per bounce
+            wocc[h] = scnt[h] + max_depth;
+            HIP_TRY(c, launch_fill_u32(cnt[h], 1u, np, hs[h]));
+            GenArgs g{};
+            g.cam = *cam;
+            g.width = width;
+            g.height = height;
+            g.wh = wh;
+            g.frame0 = f0 + fo[h];
+            g.n_paths = np;
+            g.fo = fo[h];
+            g.po = po[h];
+            g.pn = pn[h];
+            g.A = (float4 *)c->A[0].p + base[h];
+            g.B = (float4 *)c->B[0].p + base[h];
+            g.C = (float4 *)c->C[0].p + base[h];
+            g.samples = nee ? (float4 *)c->samples.p : nullptr;
+            g.P = nee ? (float *)c->P[0].p + base[h] : nullptr;
+            Timed t(c, "generate", hs[h]);
+            HIP_TRY(c, launch_generate(g, hs[h]));
         }
         for (uint32_t b = 0; b < max_depth; ++b) {
             const int cur = b & 1, nxt = cur ^ 1;
-            ExtArgs e{};
-            e.scene = c->scene;
-            e.A = (const float4 *)c->A[cur].p;
-            e.B = (const float4 *)c->B[cur].p;
-            e.H = (float4 *)c->H.p;
-            e.count = cnt + b;
-            e.work = wext + b;
-            e.ovf = ovf;
-            e.ovf_stride = g_ext * (uint32_t)trace_block();
-            {
-                Timed t(c, "extend", s);
-                HIP_TRY(c, launch_extend(e, g_ext, s));
+            for (uint32_t h = 0; h < nh; ++h) {
+                const size_t bs = base[h];
+                ExtArgs e{};
+                e.scene = c->scene;
+                e.A = (const float4 *)c->A[cur].p + bs;
+                e.B = (const float4 *)c->B[cur].p + bs;
+                e.H = (float4 *)c->H.p + bs;
+                e.count = cnt[h] + b;
+                e.work = wext[h] + b;
+                e.ovf = ovf[h];
+                e.ovf_stride = g_ext * (uint32_t)trace_block();
+                Timed t(c, "extend", hs[h]);
+                HIP_TRY(c, launch_extend(e, g_ext, hs[h]));
             }
-            ShadeArgs sh{};
-            sh.scene = c->scene;
-            sh.A = (const float4 *)c->A[cur].p;
-            sh.B = (const float4 *)c->B[cur].p;
-            sh.C = (const float4 *)c->C[cur].p;
-            sh.H = (const float4 *)c->H.p;
-            sh.A_out = (float4 *)c->A[nxt].p;
-            sh.B_out = (float4 *)c->B[nxt].p;
-            sh.C_out = (float4 *)c->C[nxt].p;
-            sh.count_in = cnt + b;
-            sh.count_out = cnt + b + 1;
-            sh.work = wsh + b;
-            sh.samples = (float4 *)c->samples.p;
-            const bool has_last = last_frame >= f0 && last_frame < f0 + nf;
-            sh.gbuf = (has_last && b == 0) ? (float4 *)out_gbuf_dev : nullptr;
-            sh.albedo = (has_last && b == 0) ? (float4 *)out_albedo_dev : nullptr;
-            sh.gbuf_frame_index = last_frame - f0;
-            sh.wh = wh;
-            sh.max_depth = max_depth;
-            if (nee) {
-                sh.nee = 1u;
-                sh.P = (const float *)c->P[cur].p;
-                sh.P_out = (float *)c->P[nxt].p;
-                sh.S_o = (float4 *)c->S_o.p;
-                sh.S_d = (float4 *)c->S_d.p;
-                sh.S_c = (float4 *)c->S_c.p;
-                sh.shadow_count = scnt + b;
+            for (uint32_t h = 0; h < nh; ++h) {
+                const uint32_t np = pn[h] * fn[h];
+                const size_t bs = base[h];
+                ShadeArgs sh{};
+                sh.scene = c->scene;
+                sh.A = (const float4 *)c->A[cur].p + bs;
+                sh.B = (const float4 *)c->B[cur].p + bs;
+                sh.C = (const float4 *)c->C[cur].p + bs;
+                sh.H = (const float4 *)c->H.p + bs;
+                sh.A_out = (float4 *)c->A[nxt].p + bs;
+                sh.B_out = (float4 *)c->B[nxt].p + bs;
+                sh.C_out = (float4 *)c->C[nxt].p + bs;
+                sh.count_in = cnt[h] + b;
+                sh.count_out = cnt[h] + b + 1;
+                sh.work = wsh[h] + b;
+                sh.samples = (float4 *)c->samples.p;
+                sh.gbuf = (has_last && b == 0) ? (float4 *)out_gbuf_dev : nullptr;
+                sh.albedo = (has_last && b == 0) ? (float4 *)out_albedo_dev : nullptr;
+                sh.gbuf_frame_index = last_frame - f0;
+                sh.wh = wh;
+                sh.max_depth = max_depth;
+                if (nee) {
+                    sh.nee = 1u;
+                    sh.P = (const float *)c->P[cur].p + bs;
+                    sh.P_out = (float *)c->P[nxt].p + bs;
+                    sh.S_o = (float4 *)c->S_o.p + bs;
+                    sh.S_d = (float4 *)c->S_d.p + bs;
+                    sh.S_c = (float4 *)c->S_c.p + bs;
+                    sh.shadow_count = scnt[h] + b;
+                }
+                {
+                    Timed t(c, "shade", hs[h]);
+                    HIP_TRY(c, launch_shade(sh, (np + 255u) / 256u, hs[h]));
+                }
+                if (nee) {
+                    OccArgs oa{};
+                    oa.scene = c->scene;
+                    oa.S_o = (const float4 *)c->S_o.p + bs;
+                    oa.S_d = (const float4 *)c->S_d.p + bs;
+                    oa.S_c = (const float4 *)c->S_c.p + bs;
+                    oa.count = scnt[h] + b;
+                    oa.work = wocc[h] + b;
+                    oa.samples = (float4 *)c->samples.p;
+                    oa.ovf = ovf[h];
+                    oa.ovf_stride = g_occ * (uint32_t)trace_block();
+                    Timed t(c, "occlude", hs[h]);
+                    HIP_TRY(c, launch_occlude(oa, g_occ, hs[h]));
+                }
             }
-            {
-                Timed t(c, "shade", s);
-                HIP_TRY(c, launch_shade(sh, (np + 255u) / 256u, s));
-            }
-            if (nee) {
-                OccArgs oa{};
-                oa.scene = c->scene;
-                oa.S_o = (const float4 *)c->S_o.p;
-                oa.S_d = (const float4 *)c->S_d.p;
-                oa.S_c = (const float4 *)c->S_c.p;
-                oa.count = scnt + b;
-                oa.work = wocc + b;
-                oa.samples = (float4 *)c->samples.p;
-                oa.ovf = ovf;
-                oa.ovf_stride = g_occ * (uint32_t)trace_block();
-                Timed t(c, "occlude", s);
-                HIP_TRY(c, launch_occlude(oa, g_occ, s));
-            }
+        }
+        if (two) {
+            HIP_TRY(c, hipEventRecord(c->ev_join, hs[1]));
+            HIP_TRY(c, hipStreamWaitEvent(s, c->ev_join, 0));
         }
         AccArgs acc{};
         acc.samples = (const float4 *)c->samples.p;
@@ -733,6 +788,7 @@ extern "C" int mcrt_render_aux_dev(mcrt_ctx *c, const CameraUniform *cam, uint32
             HIP_TRY(c, launch_accumulate(acc, s));
         }
     }
+    c->last_halves = halves;
     c->last_batches = n_batches;
     c->last_depth = max_depth;
     c->last_samples = (uint64_t)wh * spp;
@@ -772,10 +828,11 @@ extern "C" int mcrt_render_counters(mcrt_ctx *c, uint64_t *segments, uint64_t *s
     if (c->last_batches == 0) return fail(c, MCRT_ERR_NO_SCENE, "no render has run on this context");
     HIP_TRY(c, hipStreamSynchronize(c->last_stream));
     const uint32_t per_batch = counters_per_batch(c->last_depth);
-    std::vector<uint32_t> h((size_t)per_batch * c->last_batches);
+    const uint32_t blocks = c->last_batches * c->last_halves;  // counter blocks, one per batch half
+    std::vector<uint32_t> h((size_t)per_batch * blocks);
     HIP_TRY(c, hipMemcpy(h.data(), c->counters.p, h.size() * 4, hipMemcpyDeviceToHost));
     uint64_t seg = 0;
-    for (uint32_t bi = 0; bi < c->last_batches; ++bi)
+    for (uint32_t bi = 0; bi < blocks; ++bi)
         for (uint32_t b = 0; b < c->last_depth; ++b) seg += h[(size_t)bi * per_batch + b];
     if (segments) *segments = seg;
     if (samples) *samples = c->last_samples;
@@ -811,7 +868,7 @@ extern "C" int mcrt_trace_closest_dev(mcrt_ctx *c, const float *origins_dev, con
     uint32_t need = (n + 63u) / 64u;  // waves of work
     uint32_t waves_per_block = (uint32_t)trace_block() / 64u;
     grid = std::max<uint32_t>(1, std::min(grid, (need + waves_per_block - 1) / waves_per_block));
-    if ((rc = ensure_overflow(c, grid * (uint32_t)trace_block(), c->scene.stack_cull != 0, &a.ovf))) return rc;
+    if ((rc = ensure_overflow(c, grid * (uint32_t)trace_block(), c->scene.stack_cull != 0, 0, &a.ovf))) return rc;
     a.ovf_stride = grid * (uint32_t)trace_block();
     {
         Timed t(c, "trace_closest", s);
@@ -948,7 +1005,7 @@ int diff_setup(mcrt_ctx *c, const CameraUniform *cam, uint32_t width, uint32_t h
     }
     const uint32_t grid = diff_grid(width * height);
     a.ovf_stride = grid * 256u;
-    int rc = ensure_overflow(c, a.ovf_stride, false, &a.ovf);
+    int rc = ensure_overflow(c, a.ovf_stride, false, 0, &a.ovf);
     if (rc) return rc;
     return MCRT_OK;
 }

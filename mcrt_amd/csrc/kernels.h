@@ -13,7 +13,11 @@ struct GenArgs {
     CameraUniform cam;
     uint32_t width, height, wh;
     uint32_t frame0;   // frame of path 0 (frame_count of the first frame)
-    uint32_t n_paths;  // wh * frames in this batch
+    uint32_t n_paths;  // pixels * frames generated by this launch
+    // This launch's share of the batch: frames fo .. fo + n_paths / pn - 1
+    // (batch-relative) of pixels po .. po + pn - 1.  Path id = (fo + frame) * wh
+    // + pixel: frame-major over the whole batch whatever the split.
+    uint32_t fo, po, pn;
     float4 *A, *B, *C;
     float4 *samples;   // next-event estimation: zeroed per path (else null)
     float *P;          // next-event estimation: BSDF pdf of the arriving ray, 0 = camera

@@ -36,10 +36,10 @@ namespace mcrt {
 __global__ __launch_bounds__(256) void k_generate(GenArgs a) {
     uint32_t p = blockIdx.x * 256u + threadIdx.x;
     if (p >= a.n_paths) return;
-    const uint32_t nf = a.n_paths / a.wh;  // frames in this batch
-    const uint32_t pix = p / nf, fi = p % nf;
+    const uint32_t nf = a.n_paths / a.pn;  // frames in this launch
+    const uint32_t pix = a.po + p / nf, fi = p % nf;
     const uint32_t x = pix % a.width, y = pix / a.width;
-    const uint32_t pid = fi * a.wh + pix;
+    const uint32_t pid = (a.fo + fi) * a.wh + pix;
     uint32_t rng;
     V3 ro, rd;
     camera_ray(a.cam, a.width, a.height, x, y, a.frame0 + fi, rng, ro, rd);
