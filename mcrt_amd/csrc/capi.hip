@@ -719,6 +719,13 @@ extern "C" int mcrt_render_aux_dev(mcrt_ctx *c, const CameraUniform *cam, uint32
                 e.work = wext[h] + b;
                 e.ovf = ovf[h];
                 e.ovf_stride = g_ext * (uint32_t)trace_block();
+                if (nee && b > 0) {  // the previous bounce's shadow rays ride along
+                    e.S_o = (const float4 *)c->S_o.p + bs;
+                    e.S_d = (const float4 *)c->S_d.p + bs;
+                    e.S_c = (const float4 *)c->S_c.p + bs;
+                    e.occ_count = scnt[h] + (b - 1);
+                    e.samples = (float4 *)c->samples.p;
+                }
                 Timed t(c, "extend", hs[h]);
                 HIP_TRY(c, launch_extend(e, g_ext, hs[h]));
             }
@@ -756,7 +763,7 @@ extern "C" int mcrt_render_aux_dev(mcrt_ctx *c, const CameraUniform *cam, uint32
                     Timed t(c, "shade", hs[h]);
                     HIP_TRY(c, launch_shade(sh, (np + 255u) / 256u, hs[h]));
                 }
-                if (nee) {
+                if (nee && b + 1 == max_depth) {  // the last bounce's shadow rays
                     OccArgs oa{};
                     oa.scene = c->scene;
                     oa.S_o = (const float4 *)c->S_o.p + bs;

@@ -27,6 +27,11 @@ struct ExtArgs {
     DevScene scene;
     const float4 *A, *B;
     float4 *H;
+    // Optional shadow rays traced in the same launch, queued after the *count
+    // extension rays (null occ_count: none); layout as OccArgs.
+    const float4 *S_o, *S_d, *S_c;
+    const uint32_t *occ_count;
+    float4 *samples;
     const uint32_t *count;
     uint32_t *work;
     uint32_t *ovf;  // traversal-stack overflow, (depth - short) x ovf_stride

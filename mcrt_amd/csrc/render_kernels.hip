@@ -70,6 +70,48 @@ struct ExtIO {
     }
 };
 
+// Extension rays [0, n_ext) followed by the previous bounce's shadow rays in
+// one launch (next-event estimation: one persistent drain per bounce instead
+// of two).  A shadow ray adds its contribution to its path's sample when
+// nothing lies before the light, as k_occlude does.
+struct ExtOccIO {
+    const ExtArgs &a;
+    uint32_t n_ext;
+    __device__ uint32_t count() const { return n_ext + *a.occ_count; }
+    __device__ uint32_t *work() const { return a.work; }
+    __device__ void load(uint32_t slot, V3 &o, V3 &d, float &tmin, float &tmax) const {
+        tmin = MCRT_T_MIN;
+        if (slot < n_ext) {
+            float4 A = a.A[slot], B = a.B[slot];
+            o = xyz(A);
+            d = xyz(B);
+            tmax = MCRT_T_MAX;
+        } else {
+            const float4 so = a.S_o[slot - n_ext], sd = a.S_d[slot - n_ext];
+            o = xyz(so);
+            d = xyz(sd);
+            tmax = sd.w;
+        }
+    }
+    __device__ void store(uint32_t slot, const Best &b) const {
+        if (slot < n_ext) {
+            a.H[slot] = b.prim != kNone ? make_float4(b.t, b.u, b.v, __uint_as_float(b.prim))
+                                        : make_float4(-1.0f, 0.0f, 0.0f, __uint_as_float(kNone));
+        } else if (b.prim == kNone) {
+            const uint32_t i = slot - n_ext;
+            const uint32_t pid = __float_as_uint(a.S_o[i].w);
+            const float4 c = a.S_c[i], s = a.samples[pid];
+            a.samples[pid] = make_float4(s.x + c.x, s.y + c.y, s.z + c.z, 0.0f);
+        }
+    }
+};
+
+template <int kBlock, bool kSpheres, bool kCull>
+__global__ __launch_bounds__(kBlock) void k_extend_occ(ExtArgs a) {
+    __shared__ uint32_t lds[TStack<kCull>::kShort * TStack<kCull>::kWords * kBlock];
+    trace_persistent<kBlock, kSpheres, kCull>(a.scene, ExtOccIO{a, *a.count}, lds, a.ovf, a.ovf_stride);
+}
+
 // The sphere-free, plain-stack variant (cache-resident scenes) fits 64 VGPRs
 // and runs 8 waves per SIMD (C2: +1.2% over 7); the others keep the
 // compiler's choice (forcing 8 there spills, or costs cache on large scenes).
@@ -315,6 +357,10 @@ struct ExtendK {
     static const void *fn() { return reinterpret_cast<const void *>(&k_extend<kBlock, kSpheres, kCull>); }
 };
 template <int kBlock, bool kSpheres, bool kCull>
+struct ExtendOccK {
+    static const void *fn() { return reinterpret_cast<const void *>(&k_extend_occ<kBlock, kSpheres, kCull>); }
+};
+template <int kBlock, bool kSpheres, bool kCull>
 struct TraceK {
     static const void *fn() { return reinterpret_cast<const void *>(&k_trace<kBlock, kSpheres, kCull>); }
 };
@@ -338,6 +384,7 @@ hipError_t launch_traversal(const void *fn, const Args &a, uint32_t grid, hipStr
 }
 
 hipError_t launch_extend(const ExtArgs &a, uint32_t grid, hipStream_t s) {
+    if (a.occ_count) return launch_traversal(traversal_kernel<ExtendOccK>(a.scene), a, grid, s);
     return launch_traversal(traversal_kernel<ExtendK>(a.scene), a, grid, s);
 }
 
