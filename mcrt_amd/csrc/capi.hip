@@ -761,7 +761,7 @@ extern "C" int mcrt_render_aux_dev(mcrt_ctx *c, const CameraUniform *cam, uint32
                 }
                 {
                     Timed t(c, "shade", hs[h]);
-                    HIP_TRY(c, launch_shade(sh, (np + 255u) / 256u, hs[h]));
+                    HIP_TRY(c, launch_shade(sh, (np + kShadeBlock - 1u) / kShadeBlock, hs[h]));
                 }
                 if (nee && b + 1 == max_depth) {  // the last bounce's shadow rays
                     OccArgs oa{};

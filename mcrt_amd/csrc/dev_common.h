@@ -196,9 +196,14 @@ __device__ __forceinline__ V3 tri_normal(const DevScene &sc, uint32_t ordered, f
 // independent of visit order, so near-first ordering, the stack layout and
 // the scheduling of rays onto lanes leave every result unchanged.
 //
-// Scheduling: a wave pulls chunks of kChunk ray slots with one atomic and
-// hands them out to idle lanes whenever at least DevScene::refill lanes are
-// idle, so lanes whose rays finished early are refilled instead of idling
+// Scheduling: every wave's first chunk is fixed by its index, min(kChunk,
+// ceil(count / waves)) slots, so a launch starts without touching the shared
+// counter and a queue of up to kChunk slots per wave needs no atomic at all.
+// After that a wave pulls chunks of kChunk slots with one atomic.  (The
+// counter is a single device-scope word that every wave of the grid would
+// otherwise hit at once on entry; those pulls queue behind each other.)  A
+// wave hands its slots to idle lanes whenever at least DevScene::refill lanes
+// are idle, so lanes whose rays finished early are refilled instead of idling
 // until the slowest ray of the wave ends.  Inside an iteration every lane first walks
 // internal nodes until it reaches a leaf (or finishes), then the leaves are
 // intersected together (the "while-while" order).
@@ -418,8 +423,13 @@ __device__ __forceinline__ void trace_persistent(const DevScene &sc, const IO &i
     const uint32_t lane = __lane_id();
     TStack<kCull> st{lds, ovf, ovf_stride, threadIdx.x, blockIdx.x * kBlock + threadIdx.x, (uint32_t)kBlock};
     const uint32_t total = io.count();
-    uint32_t wbase = 0, wend = 0;  // wave-local slot range
-    bool more = true;
+    const uint32_t waves = gridDim.x * (uint32_t)(kBlock / 64);
+    const uint32_t wave_id = blockIdx.x * (uint32_t)(kBlock / 64) + threadIdx.x / 64u;
+    const uint32_t first = min(kChunk, (total + waves - 1u) / waves);  // static first chunk
+    const uint32_t dyn0 = first * waves;                                // first pulled slot
+    uint32_t wbase = min(wave_id * first, total);                       // wave-local slot range
+    uint32_t wend = min(wbase + first, total);
+    bool more = dyn0 < total;
     // lane state
     uint32_t slot = kNone;
     V3 o = v3(0, 0, 0), d = v3(0, 0, 0), inv = v3(0, 0, 0);
@@ -439,7 +449,7 @@ __device__ __forceinline__ void trace_persistent(const DevScene &sc, const IO &i
         if (nidle >= sc.refill) {
             if (wbase >= wend && more) {
                 uint32_t g = 0;
-                if (lane == 0) g = atomicAdd(io.work(), kChunk);
+                if (lane == 0) g = dyn0 + atomicAdd(io.work(), kChunk);
                 g = __shfl(g, 0);
                 if (g < total) {
                     wbase = g;

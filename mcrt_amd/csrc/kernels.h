@@ -38,6 +38,11 @@ struct ExtArgs {
     uint32_t ovf_stride;  // threads in the persistent grid
 };
 
+// Threads per k_shade block.  Survivors are enqueued with one atomic per
+// block on a single device-scope counter, so larger blocks mean fewer of
+// those serialised atomics.
+constexpr uint32_t kShadeBlock = 512;
+
 struct ShadeArgs {
     DevScene scene;
     const float4 *A, *B, *C, *H;

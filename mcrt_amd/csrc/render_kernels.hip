@@ -128,11 +128,12 @@ void k_extend(ExtArgs a) {
 // blocks past the live count return at once.  Survivors are compacted into
 // the next queue with one atomic per block.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_shade(ShadeArgs a) {
-    __shared__ uint32_t s_wave[4], s_swave[4];
+__global__ __launch_bounds__(kShadeBlock) void k_shade(ShadeArgs a) {
+    constexpr uint32_t kWaves = kShadeBlock / 64u;
+    __shared__ uint32_t s_wave[kWaves], s_swave[kWaves];
     __shared__ uint32_t s_base, s_sbase;
     const uint32_t count = *a.count_in;
-    const uint32_t base = blockIdx.x * 256u;
+    const uint32_t base = blockIdx.x * kShadeBlock;
     if (base >= count) return;
     const uint32_t slot = base + threadIdx.x;
     bool alive = false, shadow = false;
@@ -222,7 +223,8 @@ __global__ __launch_bounds__(256) void k_shade(ShadeArgs a) {
     if (__lane_id() == 0) s_wave[wave] = (uint32_t)__popcll(mask);
     __syncthreads();
     if (threadIdx.x == 0) {
-        uint32_t total = s_wave[0] + s_wave[1] + s_wave[2] + s_wave[3];
+        uint32_t total = 0;
+        for (uint32_t w = 0; w < kWaves; ++w) total += s_wave[w];
         s_base = total ? atomicAdd(a.count_out, total) : 0u;
     }
     __syncthreads();
@@ -241,7 +243,8 @@ __global__ __launch_bounds__(256) void k_shade(ShadeArgs a) {
     if (__lane_id() == 0) s_swave[wave] = (uint32_t)__popcll(smask);
     __syncthreads();
     if (threadIdx.x == 0) {
-        uint32_t total = s_swave[0] + s_swave[1] + s_swave[2] + s_swave[3];
+        uint32_t total = 0;
+        for (uint32_t w = 0; w < kWaves; ++w) total += s_swave[w];
         s_sbase = total ? atomicAdd(a.shadow_count, total) : 0u;
     }
     __syncthreads();
@@ -390,7 +393,7 @@ hipError_t launch_extend(const ExtArgs &a, uint32_t grid, hipStream_t s) {
 
 // `grid` = blocks covering the largest queue of the batch.
 hipError_t launch_shade(const ShadeArgs &a, uint32_t grid, hipStream_t s) {
-    hipLaunchKernelGGL(k_shade, dim3(grid), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(k_shade, dim3(grid), dim3(kShadeBlock), 0, s, a);
     return hipGetLastError();
 }
 
@@ -417,7 +420,7 @@ hipError_t occupancy(int which, const DevScene &sc, int *blocks_per_cu) {
         f = traversal_kernel<OccludeK>(sc);
     else
         f = traversal_kernel<TraceK>(sc);
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, f, which == 1 ? 256 : kTraceBlock, 0);
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, f, which == 1 ? (int)kShadeBlock : kTraceBlock, 0);
 }
 
 void stack_layout(bool cull, uint32_t *short_entries, uint32_t *words_per_entry) {
