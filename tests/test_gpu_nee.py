@@ -56,6 +56,21 @@ def test_cornell_matches_oracle_nee(nee):
     assert same_g >= 0.999 and ok >= 0.99
 
 
+@pytest.mark.parametrize("spp", [1, 3])
+def test_pixel_halves_match_oracle_nee(nee, spp):
+    """Batches of 1-3 frames of a cache-resident scene run as two pixel halves
+    on two streams (odd pixel count here: the halves differ by one pixel)."""
+    W, H = 45, 31
+    w = configs.c1(width=W, height=H)
+    print(f"render c1 45x31 {spp}spp nee", flush=True)
+    nee.upload_scene(w.buffers)
+    acc, gb = nee.render(w.camera, W, H, spp, 0, 8)
+    ref, rgb = oracle.render_nee(oracle.Scene(w.buffers), w.camera, W, H, spp, 0, 8)
+    same_g, ok = _parity(acc, gb, ref, rgb)
+    assert np.all(np.isfinite(acc))
+    assert same_g >= 0.999 and ok >= 0.99
+
+
 def test_sphere_light_matches_oracle_and_closed_form(nee):
     loaded, sb = sphere_light_scene()
     cam = cam_mod.camera_for(loaded, 8, 8)
