@@ -18,9 +18,14 @@
 // sample once to get L, then replays the same PCG stream, recovering
 // R_{k+1} = (R_k - E_k) / att_k on the way out instead of storing the path.
 //
-// Layout: one thread per pixel, frames looped inside the thread.  Gradients
-// sit in registers (kMaxParams x 4), are summed per block in a fixed order
-// and then over blocks in a fixed order, so results are reproducible.
+// Layout: one thread per (frame group, pixel), the group's frames looped
+// inside the thread; threads of one wave hold neighbouring pixels of the same
+// group.  A small image is split into several frame groups so that the grid
+// holds about 2^20 threads (a few waves per SIMD slot, which balances the
+// waves' uneven path lengths).  Gradients sit in registers (kMaxParams x 4),
+// are summed per wave by a fixed butterfly, per block in wave order and then
+// over blocks in block order, so results are reproducible.  The primal sums
+// each group's frames, then k_diff_image adds the groups in order.
 #include "kernels.h"
 #include "shading.h"
 
@@ -94,15 +99,19 @@ __device__ __forceinline__ float safe_div(float x, float y) { return y != 0.0f ?
 
 template <int kBlock, bool kSpheres, bool kAdjoint>
 __global__ __launch_bounds__(kBlock) void k_diff(DiffArgs a) {
+    constexpr int kWaves = kBlock / 64;
     __shared__ uint32_t lds[Stack::kShort * Stack::kWords * kBlock];
-    __shared__ float red[kAdjoint ? kMaxParams * 4 * kBlock : 1];
-    const uint32_t pix = blockIdx.x * kBlock + threadIdx.x;
+    __shared__ float red[kAdjoint ? kMaxParams * 4 * kWaves : 1];
+    const uint32_t tid = blockIdx.x * kBlock + threadIdx.x;
     const uint32_t wh = a.width * a.height;
-    Stack st{lds, a.ovf, a.ovf_stride, threadIdx.x, pix, (uint32_t)kBlock};
+    const uint32_t grp = tid / wh, pix = tid - grp * wh;
+    const uint32_t per = (a.spp + a.groups - 1u) / a.groups;  // frames per group
+    const uint32_t i0 = grp * per, i1 = min(a.spp, i0 + per);
+    Stack st{lds, a.ovf, a.ovf_stride, threadIdx.x, tid, (uint32_t)kBlock};
     float g[kMaxParams][4];
 #pragma unroll
     for (int q = 0; q < kMaxParams; ++q) g[q][0] = g[q][1] = g[q][2] = g[q][3] = 0.0f;
-    if (pix < wh) {
+    if (grp < a.groups) {
         const uint32_t x = pix % a.width, y = pix / a.width;
         V3 sum = v3(0.0f, 0.0f, 0.0f);
         V3 delta = v3(0.0f, 0.0f, 0.0f);
@@ -111,7 +120,7 @@ __global__ __launch_bounds__(kBlock) void k_diff(DiffArgs a) {
             const float inv = 1.0f / (float)a.spp;  // dI/dL_s of the mean
             delta = v3(dI.x * inv, dI.y * inv, dI.z * inv);
         }
-        for (uint32_t i = 0; i < a.spp; ++i) {
+        for (uint32_t i = i0; i < i1; ++i) {
             uint32_t rng0;
             V3 o, d;
             camera_ray(a.cam, a.width, a.height, x, y, a.frame0 + i, rng0, o, d);
@@ -146,23 +155,50 @@ __global__ __launch_bounds__(kBlock) void k_diff(DiffArgs a) {
             }
         }
         if (!kAdjoint) {
-            const float inv = 1.0f / (float)a.spp;
-            a.image[pix] = make_float4(sum.x * inv, sum.y * inv, sum.z * inv, 1.0f);
+            if (a.groups == 1u) {
+                const float inv = 1.0f / (float)a.spp;
+                a.image[pix] = make_float4(sum.x * inv, sum.y * inv, sum.z * inv, 1.0f);
+            } else {
+                a.image_part[tid] = make_float4(sum.x, sum.y, sum.z, 0.0f);
+            }
         }
     }
     if (kAdjoint) {
-        // Fixed-order block reduction: thread j sums column j over the block.
+        // Fixed-order reduction: a butterfly over the wave's lanes, then
+        // thread j adds column j of the block's waves in wave order.
+        const uint32_t wave = threadIdx.x >> 6;
 #pragma unroll
         for (int q = 0; q < kMaxParams; ++q)
 #pragma unroll
-            for (int c = 0; c < 4; ++c) red[(q * 4 + c) * kBlock + threadIdx.x] = g[q][c];
+            for (int c = 0; c < 4; ++c) {
+                float v = g[q][c];
+#pragma unroll
+                for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off);
+                if (__lane_id() == 0) red[wave * (kMaxParams * 4) + q * 4 + c] = v;
+            }
         __syncthreads();
         if (threadIdx.x < kMaxParams * 4) {
             float s = 0.0f;
-            for (int t = 0; t < kBlock; ++t) s += red[threadIdx.x * kBlock + t];
+            for (int w = 0; w < kWaves; ++w) s += red[w * (kMaxParams * 4) + threadIdx.x];
             a.partial[(size_t)blockIdx.x * (kMaxParams * 4) + threadIdx.x] = s;
         }
     }
+}
+
+// Primal with frame groups: the mean of each pixel's group sums, added in
+// group order.
+__global__ void k_diff_image(const float4 *part, uint32_t wh, uint32_t groups, uint32_t spp, float4 *image) {
+    const uint32_t pix = blockIdx.x * 256u + threadIdx.x;
+    if (pix >= wh) return;
+    float x = 0.0f, y = 0.0f, z = 0.0f;
+    for (uint32_t g = 0; g < groups; ++g) {
+        const float4 v = part[(size_t)g * wh + pix];
+        x += v.x;
+        y += v.y;
+        z += v.z;
+    }
+    const float inv = 1.0f / (float)spp;
+    image[pix] = make_float4(x * inv, y * inv, z * inv, 1.0f);
 }
 
 // Sum of the per-block partial gradients, in block order.
@@ -176,9 +212,20 @@ __global__ void k_diff_sum(const float *partial, uint32_t n_blocks, uint32_t n_o
 
 constexpr int kDiffBlock = 256;
 
+// Frame groups per pixel: enough that the grid holds about 2^20 threads,
+// at most one frame each.
+uint32_t diff_groups(uint32_t wh, uint32_t spp) {
+    const uint32_t g = (1u << 20) / wh;
+    return g < 1u ? 1u : (g > spp ? spp : g);
+}
+
+uint32_t diff_grid(uint32_t wh, uint32_t groups) {
+    return (uint32_t)(((uint64_t)wh * groups + kDiffBlock - 1) / kDiffBlock);
+}
+
 hipError_t launch_diff(const DiffArgs &a, bool adjoint, hipStream_t s) {
     const uint32_t wh = a.width * a.height;
-    const uint32_t grid = (wh + kDiffBlock - 1) / kDiffBlock;
+    const uint32_t grid = diff_grid(wh, a.groups);
     const bool sph = a.scene.sroot != kNone;
     if (adjoint) {
         if (sph)
@@ -194,10 +241,14 @@ hipError_t launch_diff(const DiffArgs &a, bool adjoint, hipStream_t s) {
             hipLaunchKernelGGL((k_diff<kDiffBlock, true, false>), dim3(grid), dim3(kDiffBlock), 0, s, a);
         else
             hipLaunchKernelGGL((k_diff<kDiffBlock, false, false>), dim3(grid), dim3(kDiffBlock), 0, s, a);
+        if (a.groups > 1u) {
+            hipError_t e = hipGetLastError();
+            if (e != hipSuccess) return e;
+            hipLaunchKernelGGL(k_diff_image, dim3((wh + 255u) / 256u), dim3(256), 0, s, a.image_part, wh, a.groups,
+                               a.spp, a.image);
+        }
     }
     return hipGetLastError();
 }
-
-uint32_t diff_grid(uint32_t wh) { return (wh + kDiffBlock - 1) / kDiffBlock; }
 
 }  // namespace mcrt

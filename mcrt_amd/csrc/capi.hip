@@ -64,7 +64,7 @@ struct mcrt_ctx {
     DevBuf host_accum, host_gbuf, host_out, host_accum_in;
     DevBuf trace_o, trace_d, trace_id, trace_t, trace_work;
     DevBuf spatial;
-    DevBuf diff_partial, diff_grad;
+    DevBuf diff_partial, diff_grad, diff_image;
     int spatial_radius = -1;
     float spatial_sigma = 0.0f;
     // last render bookkeeping
@@ -352,7 +352,7 @@ extern "C" void mcrt_destroy(mcrt_ctx *c) {
                       &c->env, &c->A[0], &c->A[1], &c->B[0], &c->B[1], &c->C[0], &c->C[1], &c->H,
                       &c->samples, &c->counters, &c->host_accum, &c->host_gbuf, &c->host_out,
                       &c->host_accum_in, &c->trace_o, &c->trace_d, &c->trace_id, &c->trace_t,
-                      &c->trace_work, &c->spatial, &c->diff_partial, &c->diff_grad, &c->emit,
+                      &c->trace_work, &c->spatial, &c->diff_partial, &c->diff_grad, &c->diff_image, &c->emit,
                       &c->emit_cdf, &c->tri_emit, &c->sph_emit, &c->P[0], &c->P[1], &c->S_o, &c->S_d,
                       &c->S_c})
         release(*b);
@@ -1019,7 +1019,8 @@ int diff_setup(mcrt_ctx *c, const CameraUniform *cam, uint32_t width, uint32_t h
             return fail(c, MCRT_ERR_BAD_ARG, "param_of_material[%u] = %d out of range", m, q);
         a.pmap[m] = (int8_t)q;
     }
-    const uint32_t grid = diff_grid(width * height);
+    a.groups = diff_groups(width * height, spp);
+    const uint32_t grid = diff_grid(width * height, a.groups);
     a.ovf_stride = grid * 256u;
     int rc = ensure_overflow(c, a.ovf_stride, false, 0, &a.ovf);
     if (rc) return rc;
@@ -1041,6 +1042,10 @@ extern "C" int mcrt_diff_render_dev(mcrt_ctx *c, const CameraUniform *cam, uint3
         return rc;
     if (!out_image_dev) return fail(c, MCRT_ERR_BAD_ARG, "null image pointer");
     a.image = (float4 *)out_image_dev;
+    if (a.groups > 1u) {
+        if ((rc = ensure(c, c->diff_image, (size_t)a.groups * width * height * sizeof(float4)))) return rc;
+        a.image_part = (float4 *)c->diff_image.p;
+    }
     hipStream_t s = (hipStream_t)stream;
     c->last_stream = s;
     Timed t(c, "diff_primal", s);
@@ -1060,7 +1065,7 @@ extern "C" int mcrt_diff_adjoint_dev(mcrt_ctx *c, const CameraUniform *cam, uint
                          param_of_material, a)))
         return rc;
     if (!d_image_dev || !out_grad_dev) return fail(c, MCRT_ERR_BAD_ARG, "null pointer");
-    const uint32_t grid = diff_grid(width * height);
+    const uint32_t grid = diff_grid(width * height, a.groups);
     if ((rc = ensure(c, c->diff_partial, (size_t)grid * kMaxParams * 4 * sizeof(float)))) return rc;
     a.d_image = (const float4 *)d_image_dev;
     a.partial = (float *)c->diff_partial.p;

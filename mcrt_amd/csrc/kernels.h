@@ -111,9 +111,11 @@ struct DiffArgs {
     DevScene scene;
     CameraUniform cam;
     uint32_t width, height, spp, frame0, max_depth, n_params;
+    uint32_t groups;         // frame groups per pixel (one thread each); diff_groups()
     float4 params[kMaxParams];  // (albedo.rgb, emission)
     int8_t pmap[MCRT_MAX_MATERIALS];  // material slot -> parameter index or -1
     float4 *image;           // primal: W*H mean radiance
+    float4 *image_part;      // primal with groups > 1: groups x W*H radiance sums
     const float4 *d_image;   // adjoint: dLoss/dimage
     float *partial;          // adjoint: per-block sums, kMaxParams*4 per block
     float *grad;             // adjoint: n_params*4
@@ -122,7 +124,8 @@ struct DiffArgs {
 };
 
 hipError_t launch_diff(const DiffArgs &a, bool adjoint, hipStream_t s);
-uint32_t diff_grid(uint32_t wh);
+uint32_t diff_groups(uint32_t wh, uint32_t spp);
+uint32_t diff_grid(uint32_t wh, uint32_t groups);
 
 // --- learned denoiser (learned_kernels.hip) ---------------------------------
 struct PackArgs {
