@@ -21,7 +21,7 @@
 // Layout: one thread per (frame group, pixel), the group's frames looped
 // inside the thread; threads of one wave hold neighbouring pixels of the same
 // group.  A small image is split into several frame groups so that the grid
-// holds about 2^20 threads (a few waves per SIMD slot, which balances the
+// holds about 2^21 threads (several waves per SIMD slot, which balances the
 // waves' uneven path lengths).  Gradients sit in registers (kMaxParams x 4),
 // are summed per wave by a fixed butterfly, per block in wave order and then
 // over blocks in block order, so results are reproducible.  The primal sums
@@ -212,10 +212,10 @@ __global__ void k_diff_sum(const float *partial, uint32_t n_blocks, uint32_t n_o
 
 constexpr int kDiffBlock = 256;
 
-// Frame groups per pixel: enough that the grid holds about 2^20 threads,
+// Frame groups per pixel: enough that the grid holds about 2^21 threads,
 // at most one frame each.
 uint32_t diff_groups(uint32_t wh, uint32_t spp) {
-    const uint32_t g = (1u << 20) / wh;
+    const uint32_t g = (1u << 21) / wh;
     return g < 1u ? 1u : (g > spp ? spp : g);
 }
 
