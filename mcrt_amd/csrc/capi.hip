@@ -659,6 +659,16 @@ extern "C" int mcrt_render_aux_dev(mcrt_ctx *c, const CameraUniform *cam, uint32
         hs[1] = c->side_stream;
     }
 
+    // Pinhole camera (zero defocus, no -0 origin component) without next-event
+    // estimation: the first k_shade recomputes each path's camera ray instead
+    // of reading it, and for cache-resident sphere-free scenes k_extend does
+    // too, so k_generate is skipped (CamGen).
+    bool pinhole = true;
+    for (int k = 0; k < 3; ++k)
+        pinhole = pinhole && cam->defocus_u[k] == 0.0f && cam->defocus_v[k] == 0.0f &&
+                  !(cam->origin[k] == 0.0f && std::signbit(cam->origin[k]));
+    const bool shade_gen = pinhole && !nee;
+    const bool ext_gen = shade_gen && c->scene.stack_cull == 0 && c->scene.sroot == kNone;
     const uint32_t last_frame = first_frame + spp - 1u;
     for (uint32_t bi = 0; bi < n_batches; ++bi) {
         const uint32_t f0 = first_frame + bi * F;
@@ -689,6 +699,7 @@ extern "C" int mcrt_render_aux_dev(mcrt_ctx *c, const CameraUniform *cam, uint32
             HIP_TRY(c, hipStreamWaitEvent(hs[1], c->ev_fork, 0));
         }
         uint32_t *cnt[2], *wext[2], *wsh[2], *scnt[2], *wocc[2];
+        CamGen cg[2] = {};
         for (uint32_t h = 0; h < nh; ++h) {
             const uint32_t np = pn[h] * fn[h];
             cnt[h] = counters + ((size_t)bi * halves + h) * per_batch;  // [max_depth+1] path counts
@@ -712,6 +723,19 @@ extern "C" int mcrt_render_aux_dev(mcrt_ctx *c, const CameraUniform *cam, uint32
             g.C = (float4 *)c->C[0].p + base[h];
             g.samples = nee ? (float4 *)c->samples.p : nullptr;
             g.P = nee ? (float *)c->P[0].p + base[h] : nullptr;
+            if (shade_gen) {
+                cg[h].cam = *cam;
+                cg[h].width = width;
+                cg[h].height = height;
+                cg[h].wh = wh;
+                cg[h].frame0 = g.frame0;
+                cg[h].nf = fn[h];
+                cg[h].fo = fo[h];
+                cg[h].po = po[h];
+                cg[h].on = 1u;
+                if (ext_gen) continue;
+                g.C = nullptr;
+            }
             Timed t(c, "generate", hs[h]);
             HIP_TRY(c, launch_generate(g, hs[h]));
         }
@@ -721,6 +745,7 @@ extern "C" int mcrt_render_aux_dev(mcrt_ctx *c, const CameraUniform *cam, uint32
                 const size_t bs = base[h];
                 ExtArgs e{};
                 e.scene = c->scene;
+                if (b == 0 && ext_gen) e.gen = cg[h];
                 e.A = (const float4 *)c->A[cur].p + bs;
                 e.B = (const float4 *)c->B[cur].p + bs;
                 e.H = (float4 *)c->H.p + bs;
@@ -743,6 +768,7 @@ extern "C" int mcrt_render_aux_dev(mcrt_ctx *c, const CameraUniform *cam, uint32
                 const size_t bs = base[h];
                 ShadeArgs sh{};
                 sh.scene = c->scene;
+                if (b == 0) sh.gen = cg[h];
                 sh.A = (const float4 *)c->A[cur].p + bs;
                 sh.B = (const float4 *)c->B[cur].p + bs;
                 sh.C = (const float4 *)c->C[cur].p + bs;

@@ -18,13 +18,29 @@ struct GenArgs {
     // (batch-relative) of pixels po .. po + pn - 1.  Path id = (fo + frame) * wh
     // + pixel: frame-major over the whole batch whatever the split.
     uint32_t fo, po, pn;
-    float4 *A, *B, *C;
+    float4 *A, *B, *C;  // C null: the first k_shade recomputes the throughput
     float4 *samples;   // next-event estimation: zeroed per path (else null)
     float *P;          // next-event estimation: BSDF pdf of the arriving ray, 0 = camera
 };
 
+// Camera rays of a first-bounce queue, recomputed from the queue slot instead
+// of being written by k_generate and read back (pinhole cameras, renders
+// without next-event estimation).  k_shade recomputes the ray, PCG state and
+// unit throughput (camera_ray), so k_generate writes no throughput; the
+// sphere-free plain-stack k_extend recomputes the ray too (pinhole_ray), and
+// k_generate is then skipped.  Slot order as k_generate's: pixel-major, nf
+// frames per pixel, of pixels po.. and frames fo.. of the batch; frame0 =
+// frame_count of the launch's first frame.  GenArgs::C may be null.
+struct CamGen {
+    CameraUniform cam;
+    uint32_t width, height, wh;
+    uint32_t frame0, nf, fo, po;
+    uint32_t on;  // nonzero: the queue is this launch's camera rays
+};
+
 struct ExtArgs {
     DevScene scene;
+    CamGen gen;
     const float4 *A, *B;
     float4 *H;
     // Optional shadow rays traced in the same launch, queued after the *count
@@ -45,6 +61,7 @@ constexpr uint32_t kShadeBlock = 512;
 
 struct ShadeArgs {
     DevScene scene;
+    CamGen gen;
     const float4 *A, *B, *C, *H;
     float4 *A_out, *B_out, *C_out;
     const uint32_t *count_in;

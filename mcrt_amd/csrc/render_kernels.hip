@@ -33,19 +33,27 @@ namespace mcrt {
 // so a wave starts from a few neighbouring pixels, and the bounce queues keep
 // that order through compaction.  The path id stays frame * wh + row-major
 // pixel.
+__device__ __forceinline__ void cam_path(const CameraUniform &cam, uint32_t width, uint32_t height, uint32_t wh,
+                                         uint32_t frame0, uint32_t nf, uint32_t fo, uint32_t po, uint32_t p,
+                                         uint32_t &pid, uint32_t &rng, V3 &ro, V3 &rd) {
+    const uint32_t pix = po + p / nf, fi = p % nf;
+    pid = (fo + fi) * wh + pix;
+    camera_ray(cam, width, height, pix % width, pix / width, frame0 + fi, rng, ro, rd);
+}
+
+__device__ __forceinline__ void cam_path(const CamGen &g, uint32_t p, uint32_t &pid, uint32_t &rng, V3 &ro, V3 &rd) {
+    cam_path(g.cam, g.width, g.height, g.wh, g.frame0, g.nf, g.fo, g.po, p, pid, rng, ro, rd);
+}
+
 __global__ __launch_bounds__(256) void k_generate(GenArgs a) {
     uint32_t p = blockIdx.x * 256u + threadIdx.x;
     if (p >= a.n_paths) return;
-    const uint32_t nf = a.n_paths / a.pn;  // frames in this launch
-    const uint32_t pix = a.po + p / nf, fi = p % nf;
-    const uint32_t x = pix % a.width, y = pix / a.width;
-    const uint32_t pid = (a.fo + fi) * a.wh + pix;
-    uint32_t rng;
+    uint32_t pid, rng;
     V3 ro, rd;
-    camera_ray(a.cam, a.width, a.height, x, y, a.frame0 + fi, rng, ro, rd);
+    cam_path(a.cam, a.width, a.height, a.wh, a.frame0, a.n_paths / a.pn, a.fo, a.po, p, pid, rng, ro, rd);
     a.A[p] = make_float4(ro.x, ro.y, ro.z, __uint_as_float(pid));
     a.B[p] = make_float4(rd.x, rd.y, rd.z, __uint_as_float(rng));
-    a.C[p] = make_float4(1.0f, 1.0f, 1.0f, __uint_as_float(0u));
+    if (a.C) a.C[p] = make_float4(1.0f, 1.0f, 1.0f, __uint_as_float(0u));
     if (a.samples) a.samples[pid] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     if (a.P) a.P[p] = 0.0f;
 }
@@ -53,14 +61,26 @@ __global__ __launch_bounds__(256) void k_generate(GenArgs a) {
 // ---------------------------------------------------------------------------
 // k_extend: closest hit of every queued path (world_hit at [T_MIN, T_MAX))
 // ---------------------------------------------------------------------------
+// kGen: the launch may recompute first-bounce camera rays (a.gen.on).  Only
+// the sphere-free plain-stack kernel (cache-resident scenes) does: the camera
+// arithmetic at the refill point costs registers, which the culling-stack
+// kernel of large scenes cannot spare (C3 -2%, against +2% on C2).
+template <bool kGen>
 struct ExtIO {
     const ExtArgs &a;
     __device__ uint32_t count() const { return *a.count; }
     __device__ uint32_t *work() const { return a.work; }
     __device__ void load(uint32_t slot, V3 &o, V3 &d, float &tmin, float &tmax) const {
-        float4 A = a.A[slot], B = a.B[slot];
-        o = xyz(A);
-        d = xyz(B);
+        if (kGen && a.gen.on) {  // first bounce of a pinhole camera
+            const CamGen &g = a.gen;
+            const uint32_t pix = g.po + slot / g.nf, fi = slot % g.nf;
+            uint32_t rng;
+            pinhole_ray(g.cam, g.width, g.height, pix % g.width, pix / g.width, g.frame0 + fi, rng, o, d);
+        } else {
+            float4 A = a.A[slot], B = a.B[slot];
+            o = xyz(A);
+            d = xyz(B);
+        }
         tmin = MCRT_T_MIN;
         tmax = MCRT_T_MAX;
     }
@@ -119,7 +139,7 @@ template <int kBlock, bool kSpheres, bool kCull>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu((kSpheres || kCull) ? 1 : 8, 8)))
 void k_extend(ExtArgs a) {
     __shared__ uint32_t lds[TStack<kCull>::kShort * TStack<kCull>::kWords * kBlock];
-    trace_persistent<kBlock, kSpheres, kCull>(a.scene, ExtIO{a}, lds, a.ovf, a.ovf_stride);
+    trace_persistent<kBlock, kSpheres, kCull>(a.scene, ExtIO<!kSpheres && !kCull>{a}, lds, a.ovf, a.ovf_stride);
 }
 
 // ---------------------------------------------------------------------------
@@ -140,7 +160,20 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(ShadeArgs a) {
     float4 nA, nB, nC, sO, sD, sC;
     float n_pdf = 0.0f;
     if (slot < count) {
-        float4 A = a.A[slot], B = a.B[slot], C = a.C[slot], H = a.H[slot];
+        float4 A, B, C;
+        if (a.gen.on) {  // camera ray of a first-bounce queue, as k_generate writes it
+            uint32_t pid0, rng0;
+            V3 ro, rd;
+            cam_path(a.gen, slot, pid0, rng0, ro, rd);
+            A = make_float4(ro.x, ro.y, ro.z, __uint_as_float(pid0));
+            B = make_float4(rd.x, rd.y, rd.z, __uint_as_float(rng0));
+            C = make_float4(1.0f, 1.0f, 1.0f, __uint_as_float(0u));
+        } else {
+            A = a.A[slot];
+            B = a.B[slot];
+            C = a.C[slot];
+        }
+        const float4 H = a.H[slot];
         V3 o = xyz(A), d = xyz(B), thr = xyz(C);
         uint32_t pid = __float_as_uint(A.w);
         uint32_t rng = __float_as_uint(B.w);

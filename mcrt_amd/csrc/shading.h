@@ -66,6 +66,23 @@ __device__ __forceinline__ void camera_ray(const CameraUniform &c, uint32_t widt
     rd = normalize(focal - ro);
 }
 
+// camera_ray for a pinhole camera: every defocus component is zero and no
+// origin component is -0.  The lens offset lx * defocus_u + ly * defocus_v is
+// then a zero for every finite (lx, ly), and origin + that zero is the origin
+// bit for bit, so the ray is the one camera_ray returns without the lens
+// trigonometry.  The two lens draws are still consumed.
+__device__ __forceinline__ void pinhole_ray(const CameraUniform &c, uint32_t width, uint32_t height, uint32_t x,
+                                            uint32_t y, uint32_t frame, uint32_t &rng, V3 &ro, V3 &rd) {
+    rng = pcg_hash(pcg_hash(x + y * width) ^ (frame * 2654435761u));
+    float u = ((float)x + rand_f32(rng)) / (float)width;
+    float v = 1.0f - ((float)y + rand_f32(rng)) / (float)height;
+    V3 focal = (ld3(c.lower_left) + u * ld3(c.horizontal)) + v * ld3(c.vertical);
+    rand_f32(rng);
+    rand_f32(rng);
+    ro = ld3(c.origin);
+    rd = normalize(focal - ro);
+}
+
 // ---------------------------------------------------------------------------
 // Hit record of a closest hit (:282-287 spheres, :445-459 triangles)
 // ---------------------------------------------------------------------------
