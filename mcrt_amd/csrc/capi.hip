@@ -197,7 +197,6 @@ int convert_tree(mcrt_ctx *c, const char *what, const GpuBvhNode *nodes, uint32_
     if (!nodes) return fail(c, MCRT_ERR_BAD_ARG, "%s BVH: null node array", what);
     std::vector<uint32_t> dev_index(n, kNone);
     std::vector<uint8_t> covered(n_prims, 0);
-    uint32_t n_internal = 0;
     for (uint32_t i = 0; i < n; ++i) {
         const GpuBvhNode &nd = nodes[i];
         if (nd.prim_count == 0) {
@@ -206,7 +205,6 @@ int convert_tree(mcrt_ctx *c, const char *what, const GpuBvhNode *nodes, uint32_
                             "%s BVH: internal node %u has children (%u, %u) outside the pre-order "
                             "layout of %u nodes",
                             what, i, i + 1, nd.right_or_offset, n);
-            dev_index[i] = n_internal++;
         } else {
             uint64_t end = (uint64_t)nd.right_or_offset + nd.prim_count;
             if (end > n_prims)
@@ -221,10 +219,39 @@ int convert_tree(mcrt_ctx *c, const char *what, const GpuBvhNode *nodes, uint32_
             out.last[end - 1] = 1;
         }
     }
+    // Device order: the internal children of a node take consecutive indices,
+    // a pair starting at an even index, so siblings share one 128-B line (one
+    // 64-B node each) and a traversal that visits both fetches the line once.
+    // Subtrees are laid out depth-first.  Unused slots stay zero and are never
+    // referenced.
+    uint32_t n_dev = 0;
+    if (nodes[0].prim_count == 0) {
+        dev_index[0] = 0;
+        n_dev = 2;  // the root alone in the first line
+        std::vector<uint32_t> todo{0u};
+        while (!todo.empty()) {
+            const uint32_t i = todo.back();
+            todo.pop_back();
+            const uint32_t a = i + 1, b = nodes[i].right_or_offset;
+            const bool ia = nodes[a].prim_count == 0, ib = nodes[b].prim_count == 0;
+            if (ia && ib) {
+                n_dev += n_dev & 1u;
+                dev_index[a] = n_dev;
+                dev_index[b] = n_dev + 1;
+                n_dev += 2;
+            } else if (ia) {
+                dev_index[a] = n_dev++;
+            } else if (ib) {
+                dev_index[b] = n_dev++;
+            }
+            if (ib) todo.push_back(b);
+            if (ia) todo.push_back(a);
+        }
+    }
     auto ref = [&](uint32_t i) -> uint32_t {
         return nodes[i].prim_count == 0 ? dev_index[i] : (kLeafBit | nodes[i].right_or_offset);
     };
-    out.nodes.assign((size_t)std::max<uint32_t>(n_internal, 1) * 4, make_float4(0, 0, 0, 0));
+    out.nodes.assign((size_t)std::max<uint32_t>(n_dev, 1) * 4, make_float4(0, 0, 0, 0));
     for (uint32_t i = 0; i < n; ++i) {
         if (nodes[i].prim_count != 0) continue;
         const GpuBvhNode &a = nodes[i + 1];
