@@ -413,7 +413,14 @@ __device__ __forceinline__ Best trace_one(const DevScene &sc, V3 o, V3 d, float 
     return mb.prim != kNone ? mb : sb;
 }
 
-// IO policy: count(), work(), load(slot, o, d, t_min, t_max), store(slot, best).
+// IO policy: count(), work(), load(slot, o, d, t_min, t_max), store(slot, best),
+// and kAnyHit with any_hit(slot).  An any-hit slot is a visibility query (a
+// shadow ray): its ray ends at the first leaf that yields a hit in
+// [t_min, t_max), and a sphere hit skips the mesh phase.  Up to the first hit
+// the traversal is the closest-hit one (best t is still t_max), so a miss is
+// reported exactly when the closest-hit traversal reports one; the hit stored
+// is some hit, not the closest.  Policies without kAnyHit compile none of it
+// (a per-lane flag in the extension kernels cost them 1%).
 // kSpheres = false compiles the variant for scenes without spheres (no
 // sphere phase state), launched when the sphere BVH is empty; kCull selects
 // the culling stack (lds holds TStack<kCull>::kShort * kWords words per lane).
@@ -436,7 +443,7 @@ __device__ __forceinline__ void trace_persistent(const DevScene &sc, const IO &i
     float t_min = 0.0f, t_q = 0.0f, t_cur = 0.0f;  // query and current-phase upper bounds
     uint32_t cur = kNone, sp = 0;
     uint32_t leaf = kNone;  // postponed leaf
-    bool sphere_phase = false;
+    bool sphere_phase = false, any = false;
     Best best, sbest;
     best.t = 0.0f;
     best.id = best.prim = kNone;
@@ -464,6 +471,7 @@ __device__ __forceinline__ void trace_persistent(const DevScene &sc, const IO &i
             if (slot == kNone && rank < avail) {
                 slot = wbase + rank;
                 io.load(slot, o, d, t_min, t_q);
+                if constexpr (IO::kAnyHit) any = io.any_hit(slot);
                 inv = v3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
                 best.prim = kNone;
                 best.id = kNone;
@@ -505,17 +513,23 @@ __device__ __forceinline__ void trace_persistent(const DevScene &sc, const IO &i
             else
                 tri_leaf(sc, leaf & ~kLeafBit, o, d, t_min, t_cur, best);
             leaf = kNone;
+            if (IO::kAnyHit && any && best.prim != kNone) {  // a visibility query is answered
+                cur = kNone;
+                sp = 0;
+            }
         }
         // ---- phase end: spheres -> mesh (world_hit, :471-479) or finish ----
         if (slot != kNone && cur == kNone && leaf == kNone) {
             if (kSpheres && sphere_phase) {
                 sbest = best;
                 sphere_phase = false;
-                t_cur = sbest.prim != kNone ? sbest.t : t_q;
-                best.prim = kNone;
-                best.id = kNone;
-                best.t = t_cur;
-                cur = enter_root(sc, false, o, inv, t_min, t_cur);
+                if (!(IO::kAnyHit && any && sbest.prim != kNone)) {
+                    t_cur = sbest.prim != kNone ? sbest.t : t_q;
+                    best.prim = kNone;
+                    best.id = kNone;
+                    best.t = t_cur;
+                    cur = enter_root(sc, false, o, inv, t_min, t_cur);
+                }
             }
             if (!sphere_phase && cur == kNone) {
                 io.store(slot, (!kSpheres || best.prim != kNone) ? best : sbest);

@@ -84,6 +84,8 @@ struct ExtIO {
         tmin = MCRT_T_MIN;
         tmax = MCRT_T_MAX;
     }
+    static constexpr bool kAnyHit = false;
+    __device__ bool any_hit(uint32_t) const { return false; }
     __device__ void store(uint32_t slot, const Best &b) const {
         a.H[slot] = b.prim != kNone ? make_float4(b.t, b.u, b.v, __uint_as_float(b.prim))
                                     : make_float4(-1.0f, 0.0f, 0.0f, __uint_as_float(kNone));
@@ -113,6 +115,8 @@ struct ExtOccIO {
             tmax = sd.w;
         }
     }
+    static constexpr bool kAnyHit = true;
+    __device__ bool any_hit(uint32_t slot) const { return slot >= n_ext; }  // the shadow rays
     __device__ void store(uint32_t slot, const Best &b) const {
         if (slot < n_ext) {
             a.H[slot] = b.prim != kNone ? make_float4(b.t, b.u, b.v, __uint_as_float(b.prim))
@@ -306,6 +310,8 @@ struct OccIO {
         tmin = MCRT_T_MIN;
         tmax = sd.w;
     }
+    static constexpr bool kAnyHit = true;
+    __device__ bool any_hit(uint32_t) const { return true; }
     __device__ void store(uint32_t i, const Best &b) const {
         if (b.prim != kNone) return;
         const uint32_t pid = __float_as_uint(a.S_o[i].w);
@@ -354,6 +360,8 @@ struct TraceIO {
         tmin = a.tmin;
         tmax = a.tmax;
     }
+    static constexpr bool kAnyHit = false;
+    __device__ bool any_hit(uint32_t) const { return false; }
     __device__ void store(uint32_t i, const Best &b) const {
         a.out_id[i] = b.prim != kNone ? b.id : MCRT_ID_MISS;
         a.out_t[i] = b.prim != kNone ? b.t : -1.0f;
