@@ -123,6 +123,34 @@ def test_sixteen_million_rays_properties(ctx, c2):
     assert np.array_equal(gt[sel].view(np.uint32), ot.view(np.uint32))
 
 
+def test_every_slot_traced_once(ctx, c2):
+    """Slots are handed out as one static chunk per wave, then pulled in
+    chunks from a counter.  An odd count above the static share must leave no
+    output slot unwritten (outputs start as a sentinel on the device), and
+    the head, the tail and a strided sample match the oracle."""
+    import torch
+
+    lo, hi = configs.scene_aabb(c2.buffers)
+    n = (1 << 24) + 17
+    o, d = scenes.microbench_rays(n, lo, hi, seed=77)
+    ctx.upload_scene(c2.buffers)
+    dev = torch.device("cuda:0")
+    to, td = torch.from_numpy(o).to(dev), torch.from_numpy(d).to(dev)
+    ids = torch.full((n,), 0x7FFFFFF0, dtype=torch.int32, device=dev)
+    ts = torch.full((n,), float("nan"), dtype=torch.float32, device=dev)
+    print("trace c2 16M+17 rays into sentinels", flush=True)
+    ctx.trace_closest_dev(to.data_ptr(), td.data_ptr(), n, 1e-4, 1e9, ids.data_ptr(), ts.data_ptr(),
+                          torch.cuda.current_stream(dev).cuda_stream)
+    torch.cuda.synchronize()
+    gid = ids.cpu().numpy().view(np.uint32)
+    gt = ts.cpu().numpy()
+    assert not np.any(gid == 0x7FFFFFF0) and not np.any(np.isnan(gt))
+    sel = np.concatenate([np.arange(0, 1000), np.arange(n - 1000, n), np.arange(1000, n - 1000, 997)])
+    oid, ot = oracle.trace(oracle.Scene(c2.buffers), o[sel], d[sel], mode="exact")
+    assert np.array_equal(gid[sel], oid)
+    assert np.array_equal(gt[sel].view(np.uint32), ot.view(np.uint32))
+
+
 def test_c3_large_scene_bit_exact(ctx):
     """The 2.16M-triangle voxel terrain (a traversal set far beyond the L2):
     ids and t must be the oracle's, bit for bit, for incoherent rays and for
