@@ -305,9 +305,9 @@ uint32_t counters_per_batch(uint32_t max_depth) {
 
 // Frames per batch: a batch holds at most 2^26 paths, 8.6 GB of path state
 // (11.8 GB with next-event estimation).  Larger batches give a wave more
-// frames of the same pixels and drain fewer persistent launches: C3 at 1080p,
-// 16 spp, goes from 2 batches to 1 (+7%); C2 at 1024^2, 64 spp, from 4 to 1
-// (+12%).
+// frames of the same pixels and drain fewer persistent launches: the 1080p
+// voxel terrain at 16 spp goes from 2 batches to 1 (+7%); the icosphere
+// Cornell scene at 1024^2, 64 spp, from 4 to 1 (+12%).
 constexpr uint64_t kMaxBatchPaths = 1ull << 26;
 
 uint32_t frames_per_batch(uint32_t wh, uint32_t spp) {
@@ -576,15 +576,15 @@ extern "C" int mcrt_upload_scene(mcrt_ctx *c, const GpuSphere *spheres, uint32_t
     // A refill stops the whole wave to load new rays.  Where the traversal
     // set (nodes + triangles + spheres) outgrows the 32 MB of L2 that load's
     // latency is long, and refilling more lanes per stop pays: at 40 idle
-    // lanes the 2.16M-triangle C3 frame renders 4.5% faster than at 16.  A
-    // cache-resident scene keeps lanes fuller instead (the 70k-triangle C2
-    // frame: 3% slower at 32, 8% at 48).
+    // lanes the 2.16M-triangle voxel terrain renders 4.5% faster than at 16.
+    // A cache-resident scene keeps lanes fuller instead (the 70k-triangle
+    // icosphere Cornell scene: 3% slower at 32, 8% at 48).
     const size_t traversal_bytes = (mt.nodes.size() + st.nodes.size()) * sizeof(float4) +
                                    (size_t)n_tris * 48 + (size_t)n_spheres * 32;
     d.refill = traversal_bytes <= (32u << 20) ? 16u : 40u;
-    // Culling pops skip node fetches that miss the L2 on such scenes (C3:
-    // +2.8%); on a cache-resident scene the bookkeeping and the shorter LDS
-    // stack cost more than they save (C2: -3.5%).
+    // Culling pops skip node fetches that miss the L2 on such scenes (voxel
+    // terrain: +2.8%); on a cache-resident scene the bookkeeping and the
+    // shorter LDS stack cost more than they save (icosphere Cornell: -3.5%).
     d.stack_cull = traversal_bytes <= (32u << 20) ? 0u : 1u;
     if (d.stack_depth > 96)
         return fail(c, MCRT_ERR_LIMIT, "BVH depth %u exceeds the traversal stack limit of 96",
@@ -656,8 +656,8 @@ extern "C" int mcrt_render_aux_dev(mcrt_ctx *c, const CameraUniform *cam, uint32
     // Cache-resident scenes trace a batch as two halves on two streams: a
     // bounce's persistent launch on one half drains its slowest rays while the
     // other half's launches fill the device.  A batch of at least 4 frames is
-    // split into frame halves (the C2 frame at 1080p: 6 spp 5% faster, 28 spp
-    // 2%), a batch of 1-3 frames into pixel halves (1 spp 11% faster, with
+    // split into frame halves (the icosphere Cornell frame at 1080p: 6 spp 5%
+    // faster, 28 spp 2%), a batch of 1-3 frames into pixel halves (1 spp 11% faster, with
     // next-event estimation 11-13%).  Scenes that spill the L2 stay on one
     // stream, and so do batches of 4+ frames with next-event estimation
     // (measured flat to slower).

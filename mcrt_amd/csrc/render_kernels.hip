@@ -64,7 +64,8 @@ __global__ __launch_bounds__(256) void k_generate(GenArgs a) {
 // kGen: the launch may recompute first-bounce camera rays (a.gen.on).  Only
 // the sphere-free plain-stack kernel (cache-resident scenes) does: the camera
 // arithmetic at the refill point costs registers, which the culling-stack
-// kernel of large scenes cannot spare (C3 -2%, against +2% on C2).
+// kernel of large scenes cannot spare (1080p voxel terrain -2%, against +2%
+// on the icosphere Cornell scene).
 template <bool kGen>
 struct ExtIO {
     const ExtArgs &a;
@@ -137,7 +138,7 @@ __global__ __launch_bounds__(kBlock) void k_extend_occ(ExtArgs a) {
 }
 
 // The sphere-free, plain-stack variant (cache-resident scenes) fits 64 VGPRs
-// and runs 8 waves per SIMD (C2: +1.2% over 7); the others keep the
+// and runs 8 waves per SIMD (icosphere Cornell: +1.2% over 7); the others keep the
 // compiler's choice (forcing 8 there spills, or costs cache on large scenes).
 template <int kBlock, bool kSpheres, bool kCull>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu((kSpheres || kCull) ? 1 : 8, 8)))
