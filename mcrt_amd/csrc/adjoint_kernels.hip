@@ -201,13 +201,21 @@ __global__ void k_diff_image(const float4 *part, uint32_t wh, uint32_t groups, u
     image[pix] = make_float4(x * inv, y * inv, z * inv, 1.0f);
 }
 
-// Sum of the per-block partial gradients, in block order.
-__global__ void k_diff_sum(const float *partial, uint32_t n_blocks, uint32_t n_out, float *out) {
-    uint32_t j = threadIdx.x;
-    if (j >= n_out) return;
-    float s = 0.0f;
-    for (uint32_t b = 0; b < n_blocks; ++b) s += partial[(size_t)b * (kMaxParams * 4) + j];
-    out[j] = s;
+// Sum of the per-block partial gradients: block j reduces output j, thread t
+// adding blocks t, t + 256, ... in order, then a fixed tree over the threads,
+// so the result is the same on every run.
+__global__ __launch_bounds__(256) void k_diff_sum(const float *partial, uint32_t n_blocks, float *out) {
+    __shared__ float s[256];
+    const uint32_t j = blockIdx.x;
+    float acc = 0.0f;
+    for (uint32_t b = threadIdx.x; b < n_blocks; b += 256u) acc += partial[(size_t)b * (kMaxParams * 4) + j];
+    s[threadIdx.x] = acc;
+    __syncthreads();
+    for (uint32_t w = 128u; w > 0u; w >>= 1) {
+        if (threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[j] = s[0];
 }
 
 constexpr int kDiffBlock = 256;
@@ -234,8 +242,8 @@ hipError_t launch_diff(const DiffArgs &a, bool adjoint, hipStream_t s) {
             hipLaunchKernelGGL((k_diff<kDiffBlock, false, true>), dim3(grid), dim3(kDiffBlock), 0, s, a);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
-        hipLaunchKernelGGL(k_diff_sum, dim3(1), dim3(kMaxParams * 4), 0, s, a.partial, grid,
-                           a.n_params * 4u, a.grad);
+        if (a.n_params > 0u)
+            hipLaunchKernelGGL(k_diff_sum, dim3(a.n_params * 4u), dim3(256), 0, s, a.partial, grid, a.grad);
     } else {
         if (sph)
             hipLaunchKernelGGL((k_diff<kDiffBlock, true, false>), dim3(grid), dim3(kDiffBlock), 0, s, a);

@@ -133,3 +133,44 @@ def optimize(ctx, inv: InverseWorkload, steps: int = 200, lr: float = 0.02, targ
             print(f"adam {it}: loss {hist[-1][0]:.4e} |albedo - hidden| max {err:.4f} mean {mean_err:.4f}",
                   flush=True)
     return params.detach().cpu().numpy(), hist
+
+
+def _main(argv=None) -> int:
+    """``python -m mcrt_amd.diff [--steps N] [--log-every K]``: the inverse
+    workload (512 x 512, 32 spp, depth 6; target at 8x the spp from an
+    independent frame range), N Adam steps, one JSON line with the timing and
+    the albedo error before and after."""
+    import argparse
+    import json
+    import time
+
+    import torch
+
+    from ._lib import Context
+
+    ap = argparse.ArgumentParser(prog="python -m mcrt_amd.diff")
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--log-every", type=int, default=0)
+    a = ap.parse_args(argv)
+    torch.cuda.init()
+    inv = inverse_workload()
+    with Context(0) as ctx:
+        ctx.upload_scene(inv.work.buffers)
+        target_spp = 8 * inv.work.spp
+        optimize(ctx, inv, steps=2, target_spp=target_spp)  # warm-up
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        _, hist = optimize(ctx, inv, steps=a.steps, target_spp=target_spp, log_every=a.log_every)
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+    print(json.dumps({"steps": a.steps, "total_s": round(el, 3), "ms_per_step": round(el / a.steps * 1e3, 3),
+                      "albedo_err_first": {"max": round(hist[0][1], 4), "mean": round(hist[0][2], 4)},
+                      "albedo_err_last": {"max": round(hist[-1][1], 4), "mean": round(hist[-1][2], 4)}}),
+          flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    import sys
+
+    sys.exit(_main())
