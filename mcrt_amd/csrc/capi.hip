@@ -700,7 +700,7 @@ extern "C" int mcrt_render_aux_dev(mcrt_ctx *c, const CameraUniform *cam, uint32
     for (uint32_t bi = 0; bi < n_batches; ++bi) {
         const uint32_t f0 = first_frame + bi * F;
         const uint32_t nf = std::min<uint32_t>(F, spp - bi * F);
-        const bool by_pixels = split && nf < 4;
+        const bool by_pixels = split && nf < 4 && wh >= 2u;  // both halves hold pixels
         const bool two = split && (nf >= 4 || by_pixels);
         const uint32_t nh = two ? 2u : 1u;
         // Half h: frames [fo[h], fo[h] + fn[h]) of pixels [po[h], po[h] + pn[h]);

@@ -90,6 +90,25 @@ def test_stream_halves_are_schedule_free(ctx):
     assert same_g >= 0.999 and ok >= 0.99
 
 
+@pytest.mark.parametrize("size", [(1, 1), (1, 3), (3, 1), (2, 2)])
+def test_tiny_images(ctx, size):
+    """Renders of a handful of pixels: pixel halves need two pixels, and a
+    single pixel stays on one stream.  First hits match the oracle exactly,
+    and a 5-frame render (frame halves) equals five chained 1-frame renders
+    bit for bit."""
+    W, H = size
+    w = configs.c2(width=W, height=H, spp=5)
+    ctx.upload_scene(w.buffers)
+    print(f"render c2 {W}x{H}", flush=True)
+    full, gfull = ctx.render(w.camera, W, H, 5, 0, 8)
+    acc = None
+    for f in range(5):
+        acc, g = ctx.render(w.camera, W, H, 1, f, 8, accum_in=acc)
+    assert np.array_equal(full.view(np.uint32), acc.view(np.uint32))
+    _, rgb = oracle.render(oracle.Scene(w.buffers), w.camera, W, H, 5, 0, 8)
+    assert np.array_equal(gfull, rgb) and np.array_equal(g, rgb)
+
+
 def test_thin_lens_camera_matches_oracle(ctx):
     """A pinhole camera's first bounce recomputes its camera rays from the
     queue slot; a camera with a lens (nonzero defocus vectors) keeps the rays
