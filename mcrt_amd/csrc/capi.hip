@@ -441,6 +441,11 @@ extern "C" int mcrt_upload_scene(mcrt_ctx *c, const GpuSphere *spheres, uint32_t
     if (rc) return rc;
     rc = convert_tree(c, "sphere", sphere_bvh, n_sphere_nodes, n_spheres, st);
     if (rc) return rc;
+    // Bytes a traversal can touch (nodes + triangles + spheres); the per-scene
+    // policies below compare it with the 32 MB L2.
+    const size_t traversal_bytes = (mt.nodes.size() + st.nodes.size()) * sizeof(float4) +
+                                   (size_t)n_tris * 48 + (size_t)n_spheres * 32;
+    const bool fits_l2 = traversal_bytes <= (32u << 20);
 
     // Triangles in leaf order: v0, e1 = v1 - v0, e2 = v2 - v0.
     std::vector<float4> dt((size_t)std::max<uint32_t>(n_tris, 1) * 3, make_float4(0, 0, 0, 0));
@@ -456,11 +461,15 @@ extern "C" int mcrt_upload_scene(mcrt_ctx *c, const GpuSphere *spheres, uint32_t
         dt[3 * (size_t)i + 2] = make_float4(p2[0] - p0[0], p2[1] - p0[1], p2[2] - p0[2], 0.0f);
     }
     // Shading records in leaf order: vertex normals + mat_idx (with flag).
-    std::vector<float4> dsh((size_t)std::max<uint32_t>(n_tris, 1) * 3, make_float4(0, 0, 0, 0));
+    // Scenes whose traversal set spills the L2 gather them from memory, so
+    // each is padded to one aligned 64-B sector (the 1080p voxel terrain's
+    // shade pass: -2.7%); cache-resident scenes keep the denser 48 B.
+    const uint32_t ts = fits_l2 ? 3u : 4u;
+    std::vector<float4> dsh((size_t)std::max<uint32_t>(n_tris, 1) * ts, make_float4(0, 0, 0, 0));
     for (uint32_t i = 0; i < n_tris; ++i) {
         for (int k = 0; k < 3; ++k) {
             const float *nn = verts[tris[i].v[k]].normal;
-            dsh[3 * (size_t)i + k] = make_float4(nn[0], nn[1], nn[2], k == 0 ? ubits(tris[i].mat_idx) : 0.0f);
+            dsh[ts * (size_t)i + k] = make_float4(nn[0], nn[1], nn[2], k == 0 ? ubits(tris[i].mat_idx) : 0.0f);
         }
     }
     std::vector<float4> ds((size_t)std::max<uint32_t>(n_spheres, 1) * 2, make_float4(0, 0, 0, 0));
@@ -560,6 +569,7 @@ extern "C" int mcrt_upload_scene(mcrt_ctx *c, const GpuSphere *spheres, uint32_t
     d.mnodes = (const float4 *)c->mnodes.p;
     d.mtris = (const float4 *)c->mtris.p;
     d.tshade = (const float4 *)c->tshade.p;
+    d.tshade_stride = ts;
     d.mroot = n_tris ? mt.root : kNone;
     std::memcpy(d.mroot_lo, mt.lo, sizeof mt.lo);
     std::memcpy(d.mroot_hi, mt.hi, sizeof mt.hi);
@@ -579,13 +589,11 @@ extern "C" int mcrt_upload_scene(mcrt_ctx *c, const GpuSphere *spheres, uint32_t
     // lanes the 2.16M-triangle voxel terrain renders 4.5% faster than at 16.
     // A cache-resident scene keeps lanes fuller instead (the 70k-triangle
     // icosphere Cornell scene: 3% slower at 32, 8% at 48).
-    const size_t traversal_bytes = (mt.nodes.size() + st.nodes.size()) * sizeof(float4) +
-                                   (size_t)n_tris * 48 + (size_t)n_spheres * 32;
-    d.refill = traversal_bytes <= (32u << 20) ? 16u : 40u;
+    d.refill = fits_l2 ? 16u : 40u;
     // Culling pops skip node fetches that miss the L2 on such scenes (voxel
     // terrain: +2.8%); on a cache-resident scene the bookkeeping and the
     // shorter LDS stack cost more than they save (icosphere Cornell: -3.5%).
-    d.stack_cull = traversal_bytes <= (32u << 20) ? 0u : 1u;
+    d.stack_cull = fits_l2 ? 0u : 1u;
     if (d.stack_depth > 96)
         return fail(c, MCRT_ERR_LIMIT, "BVH depth %u exceeds the traversal stack limit of 96",
                     d.stack_depth);

@@ -40,8 +40,10 @@ constexpr uint32_t kPrimCull = 2u;
 //   t2 = (e2.xyz, -)
 // e1 = v1 - v0 and e2 = v2 - v0 are formed on the host in FP32, the same
 // rounding the shader applies at :302-303.
-// Triangle shading record (48 B, leaf order), used for the back-face test
-// and by the shade pass:
+// Triangle shading record (leaf order, DevScene::tshade_stride float4 apart:
+// 48 B for scenes that fit the L2, 64 B with a padding float4 otherwise, so
+// that each record gathered from memory is one aligned 64-B sector), used for
+// the back-face test and by the shade pass:
 //   n0 = (normal of v0, mat_idx bits incl. the cull flag)
 //   n1 = (normal of v1, -)  n2 = (normal of v2, -)
 // Sphere (32 B, leaf order):
@@ -51,6 +53,7 @@ struct DevScene {
     const float4 *mnodes;
     const float4 *mtris;
     const float4 *tshade;
+    uint32_t tshade_stride;  // float4 per shading record: 3, or 4 (padded) for scenes that spill the L2
     uint32_t mroot;  // kNone = no mesh
     float mroot_lo[3], mroot_hi[3];
     const float4 *snodes;
@@ -179,7 +182,7 @@ __device__ __forceinline__ float sphere_t(V3 o, V3 d, float4 s0, float t_min, fl
 // path_trace.wgsl:448-450.  mat receives mat_idx with its cull flag.
 __device__ __forceinline__ V3 tri_normal(const DevScene &sc, uint32_t ordered, float u, float v,
                                          uint32_t &mat) {
-    const float4 *r = sc.tshade + 3u * ordered;
+    const float4 *r = sc.tshade + sc.tshade_stride * ordered;
     float4 a = r[0], b = r[1], c = r[2];
     mat = __float_as_uint(a.w);
     float w = 1.0f - u - v;
