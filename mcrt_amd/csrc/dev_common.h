@@ -35,7 +35,8 @@ constexpr uint32_t kNone = 0xFFFFFFFFu;
 constexpr uint32_t kPrimLast = 1u;
 constexpr uint32_t kPrimCull = 2u;
 
-// Triangle (48 B, leaf order), used by traversal:
+// Triangle (leaf order, DevScene::tri_stride float4 apart: 48 B for scenes
+// that fit the L2, 64 B with a padding float4 otherwise), used by traversal:
 //   t0 = (v0.xyz, orig_id bits)  t1 = (e1.xyz, flags bits)
 //   t2 = (e2.xyz, -)
 // e1 = v1 - v0 and e2 = v2 - v0 are formed on the host in FP32, the same
@@ -53,6 +54,7 @@ struct DevScene {
     const float4 *mnodes;
     const float4 *mtris;
     const float4 *tshade;
+    uint32_t tri_stride;     // float4 per triangle record: 3, or 4 (padded) for scenes that spill the L2
     uint32_t tshade_stride;  // float4 per shading record: 3, or 4 (padded) for scenes that spill the L2
     uint32_t mroot;  // kNone = no mesh
     float mroot_lo[3], mroot_hi[3];
@@ -328,7 +330,7 @@ __device__ __forceinline__ void node_step(const float4 *nodes, V3 o, V3 inv, flo
 __device__ __forceinline__ void tri_leaf(const DevScene &sc, uint32_t i, V3 o, V3 d, float t_min,
                                          float t_max, Best &best) {
     for (;;) {
-        const float4 *tp = sc.mtris + 3u * i;
+        const float4 *tp = sc.mtris + sc.tri_stride * i;
         float4 a = tp[0], b = tp[1], c = tp[2];
         uint32_t flags = __float_as_uint(b.w);
         float t, u, v;

@@ -79,7 +79,7 @@ __device__ __forceinline__ float light_pdf(const DevScene &sc, uint32_t e, V3 x,
         if (!(d2 > r2)) return 0.0f;
         return r0.x / (kTau * cone_one_minus_cos(r2, d2));
     }
-    const float4 *tp = sc.mtris + 3u * ref;
+    const float4 *tp = sc.mtris + sc.tri_stride * ref;
     const V3 ng = cross(xyz(tp[1]), xyz(tp[2]));
     const float cosl = fabsf(dot(ng, d)) / sqrtf(dot(ng, ng));
     if (!(cosl > 1e-7f)) return 0.0f;
@@ -110,7 +110,7 @@ __device__ __forceinline__ bool sample_light(const DevScene &sc, V3 x, uint32_t 
         ls.pdf = r0.x / (kTau * omc);
         return ls.dist > 0.0f && ls.pdf > 0.0f && ls.pdf < 3.0e38f;
     }
-    const float4 *tp = sc.mtris + 3u * ref;
+    const float4 *tp = sc.mtris + sc.tri_stride * ref;
     const V3 v0 = xyz(tp[0]), e1 = xyz(tp[1]), e2 = xyz(tp[2]);
     const float su = sqrtf(u1);
     const float bu = su * (1.0f - u2), bv = su * u2;
