@@ -39,7 +39,7 @@ MFMA_BF16_PEAK_TFLOPS = 2500.0
 # Measured bytes between L2 and memory per k_extend launch (FETCH_SIZE x 2 +
 # WRITE_SIZE x 1, factors from a 1 GiB copy in the same rocprofv3 --pmc runs),
 # per workload at its default size; profiles/traffic_extend_voxel_c3_1080p.md.
-EXTEND_TRAFFIC_BYTES = {"voxel_c3": 13.109e9}
+EXTEND_TRAFFIC_BYTES = {"voxel_c3": 12.358e9}
 
 
 def parse():
