@@ -5,9 +5,12 @@
 // wavefront pipeline over a batch of F frames x W*H pixels of paths:
 //
 //   k_generate    camera rays + PCG seeds for every (frame, pixel) path
-//   k_extend      closest hit for every queued path (persistent, wave64,
-//                 slots pulled from an atomic counter, lane refill, LDS
-//                 short stack; dev_common.h)
+//                 (for a pinhole camera without next-event estimation the
+//                 first k_shade, and in cache-resident scenes the first
+//                 k_extend, recompute them from the queue slot instead)
+//   k_extend      closest hit for every queued path (persistent, wave64, a
+//                 static first chunk per wave then slots pulled from an
+//                 atomic counter, lane refill, LDS short stack; dev_common.h)
 //   k_shade       G-buffer, emission/miss termination, material scatter,
 //                 block-aggregated enqueue of surviving paths
 //   k_accumulate  running mean over the batch's frames in frame order
