@@ -71,6 +71,29 @@ def test_pixel_halves_match_oracle_nee(nee, spp):
     assert same_g >= 0.999 and ok >= 0.99
 
 
+def test_large_scene_nee_matches_oracle(nee):
+    """Next-event estimation on a scene whose traversal set spills the L2,
+    where triangle and shading records are padded to 64 B: the voxel terrain
+    with one of its materials (134k triangles) made emissive, so light
+    samples read emitter triangles through the padded layout."""
+    W, H = 48, 27
+    w = configs.c3(width=W, height=H, spp=2)
+    mats = w.buffers.materials.copy()
+    m = mats["materials"][3]
+    m["type_pad"][0] = 3  # emissive
+    m["albedo_fuzz"][:3] = (1.0, 0.9, 0.7)
+    m["ior_pad"][0] = 4.0
+    sb = type(w.buffers)(**{**w.buffers.__dict__, "materials": mats})
+    print("render c3 48x27 2spp depth 3 nee, emissive terrain material", flush=True)
+    nee.upload_scene(sb)
+    acc, gb = nee.render(w.camera, W, H, 2, 0, 3)
+    ref, rgb = oracle.render_nee(oracle.Scene(sb), w.camera, W, H, 2, 0, 3)
+    same_g, ok = _parity(acc, gb, ref, rgb)
+    print(f"gbuf {same_g:.5f} px {ok:.5f}")
+    assert np.all(np.isfinite(acc)) and float(np.max(acc[..., :3])) > 0.0
+    assert same_g >= 0.999 and ok >= 0.99
+
+
 def test_sphere_light_matches_oracle_and_closed_form(nee):
     loaded, sb = sphere_light_scene()
     cam = cam_mod.camera_for(loaded, 8, 8)
