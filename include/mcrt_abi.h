@@ -275,8 +275,10 @@ int mcrt_build_sphere_bvh(const GpuSphere *spheres, uint32_t n_spheres,
  *  - image: W*H RGBA32F, the mean of frames first_frame..first_frame+spp-1.
  *  - d_image: W*H RGBA32F, dLoss/dimage (alpha ignored).
  *  - grad: n_params x 4 floats, dLoss/dparams, summed in a fixed order.
- * The adjoint re-traces every sample twice with its PCG stream (no tape);
- * replayed vertices need non-zero attenuation. */
+ * The adjoint traces every sample once, keeping its vertices on a per-thread
+ * tape in device memory (32 B x max_depth per thread of a grid of about 2^21
+ * threads, held by the context), then walks the path back from the tape;
+ * walked vertices need non-zero attenuation. */
 int mcrt_diff_render(mcrt_ctx *ctx, const CameraUniform *camera, uint32_t width,
                      uint32_t height, uint32_t spp, uint32_t first_frame,
                      uint32_t max_depth, const float *params, uint32_t n_params,

@@ -13,10 +13,16 @@
 //   dL/dE_p    = sum over vertices on p of T_k                 (white)
 //   dL/da_p[c] = sum over vertices on p of T_k[c] * R_{k+1}[c].
 //
-// Path replay backpropagation (Vicini, Speierer, Jakob, "Path Replay
-// Backpropagation", ACM TOG 40(4), 2021): the adjoint kernel traces each
-// sample once to get L, then replays the same PCG stream, recovering
-// R_{k+1} = (R_k - E_k) / att_k on the way out instead of storing the path.
+// Path replay (after Vicini, Speierer, Jakob, "Path Replay Backpropagation",
+// ACM TOG 40(4), 2021): the adjoint kernel traces each sample once to get L,
+// then walks the same path again, recovering R_{k+1} = (R_k - E_k) / att_k
+// on the way out.  Instead of re-tracing the path from its PCG stream (which
+// needs no memory), the first trace keeps each vertex's (E_k, att_k,
+// parameter) on a per-thread tape in device memory and the second walk reads
+// it back: 32 B per vertex, 2^21 threads x 6 vertices = 400 MB for the 512^2
+// inverse workload, in exchange for half the traversal work.  The values are
+// the ones a re-trace would recompute, so the gradient is unchanged bit for
+// bit.
 //
 // Layout: one thread per (frame group, pixel), the group's frames looped
 // inside the thread; threads of one wave hold neighbouring pixels of the same
@@ -124,16 +130,38 @@ __global__ __launch_bounds__(kBlock) void k_diff(DiffArgs a) {
             uint32_t rng0;
             V3 o, d;
             camera_ray(a.cam, a.width, a.height, x, y, a.frame0 + i, rng0, o, d);
-            const V3 L = diff_path_radiance<kSpheres>(a, o, d, rng0, st);
             if (!kAdjoint) {
-                sum = sum + L;
+                sum = sum + diff_path_radiance<kSpheres>(a, o, d, rng0, st);
                 continue;
             }
-            // Replay with the same PCG stream.
-            uint32_t rng = rng0;
+            // Trace once, keeping each vertex on this thread's tape.
+            V3 L = v3(0.0f, 0.0f, 0.0f);
+            {
+                V3 Tf = v3(1.0f, 1.0f, 1.0f);
+                uint32_t rng = rng0;
+                for (uint32_t b = 0; b < a.max_depth; ++b) {
+                    Vertex vx = diff_vertex<kSpheres>(a, o, d, rng, st);
+                    L = L + Tf * vx.emit;
+                    float4 *t = a.tape + 2u * ((size_t)b * a.tape_stride + tid);
+                    t[0] = make_float4(vx.emit.x, vx.emit.y, vx.emit.z, (float)vx.param);  // -1..7, exact
+                    t[1] = make_float4(vx.att.x, vx.att.y, vx.att.z, vx.cont ? 1.0f : 0.0f);
+                    if (!vx.cont) break;
+                    Tf = Tf * vx.att;
+                    o = vx.p;
+                    d = vx.dir;
+                }
+            }
+            // Replay the path from the tape: the values a second trace of the
+            // same PCG stream would recompute, bit for bit.
             V3 T = v3(1.0f, 1.0f, 1.0f), R = L;
             for (uint32_t b = 0; b < a.max_depth; ++b) {
-                Vertex vx = diff_vertex<kSpheres>(a, o, d, rng, st);
+                const float4 *t = a.tape + 2u * ((size_t)b * a.tape_stride + tid);
+                const float4 t0 = t[0], t1 = t[1];
+                Vertex vx;
+                vx.emit = xyz(t0);
+                vx.param = (int)t0.w;
+                vx.att = xyz(t1);
+                vx.cont = t1.w != 0.0f;
                 if (!vx.cont) break;
                 const V3 Rn = v3(safe_div(R.x - vx.emit.x, vx.att.x), safe_div(R.y - vx.emit.y, vx.att.y),
                                  safe_div(R.z - vx.emit.z, vx.att.z));
@@ -150,8 +178,6 @@ __global__ __launch_bounds__(kBlock) void k_diff(DiffArgs a) {
                 }
                 T = T * vx.att;
                 R = Rn;
-                o = vx.p;
-                d = vx.dir;
             }
         }
         if (!kAdjoint) {

@@ -64,7 +64,7 @@ struct mcrt_ctx {
     DevBuf host_accum, host_gbuf, host_out, host_accum_in;
     DevBuf trace_o, trace_d, trace_id, trace_t, trace_work;
     DevBuf spatial;
-    DevBuf diff_partial, diff_grad, diff_image;
+    DevBuf diff_partial, diff_grad, diff_image, diff_tape;
     int spatial_radius = -1;
     float spatial_sigma = 0.0f;
     // last render bookkeeping
@@ -379,7 +379,7 @@ extern "C" void mcrt_destroy(mcrt_ctx *c) {
                       &c->env, &c->A[0], &c->A[1], &c->B[0], &c->B[1], &c->C[0], &c->C[1], &c->H,
                       &c->samples, &c->counters, &c->host_accum, &c->host_gbuf, &c->host_out,
                       &c->host_accum_in, &c->trace_o, &c->trace_d, &c->trace_id, &c->trace_t,
-                      &c->trace_work, &c->spatial, &c->diff_partial, &c->diff_grad, &c->diff_image, &c->emit,
+                      &c->trace_work, &c->spatial, &c->diff_partial, &c->diff_grad, &c->diff_image, &c->diff_tape, &c->emit,
                       &c->emit_cdf, &c->tri_emit, &c->sph_emit, &c->P[0], &c->P[1], &c->S_o, &c->S_d,
                       &c->S_c})
         release(*b);
@@ -1134,6 +1134,9 @@ extern "C" int mcrt_diff_adjoint_dev(mcrt_ctx *c, const CameraUniform *cam, uint
     if (!d_image_dev || !out_grad_dev) return fail(c, MCRT_ERR_BAD_ARG, "null pointer");
     const uint32_t grid = diff_grid(width * height, a.groups);
     if ((rc = ensure(c, c->diff_partial, (size_t)grid * kMaxParams * 4 * sizeof(float)))) return rc;
+    a.tape_stride = grid * 256u;
+    if ((rc = ensure(c, c->diff_tape, (size_t)a.tape_stride * max_depth * 2 * sizeof(float4)))) return rc;
+    a.tape = (float4 *)c->diff_tape.p;
     a.d_image = (const float4 *)d_image_dev;
     a.partial = (float *)c->diff_partial.p;
     a.grad = out_grad_dev;

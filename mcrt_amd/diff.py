@@ -3,7 +3,8 @@
 * :class:`DiffRender` — a thin ``torch.autograd.Function``: forward calls
   ``mcrt_diff_render_dev``, backward calls ``mcrt_diff_adjoint_dev`` with the
   incoming image gradient.  Both run on the tensors' device memory and the
-  current stream; nothing is recorded on a tape.
+  current stream; torch records nothing but the Function's inputs (the
+  adjoint keeps its own per-thread vertex tape inside the library).
 * :func:`inverse_workload` — the displaced-icosphere scene whose 8 icosphere
   materials become parameter materials (RGB albedo + scalar emission).
 * :func:`optimize` — Adam on the L2 image loss against a target rendered with
