@@ -276,7 +276,7 @@ int mcrt_build_sphere_bvh(const GpuSphere *spheres, uint32_t n_spheres,
  *  - d_image: W*H RGBA32F, dLoss/dimage (alpha ignored).
  *  - grad: n_params x 4 floats, dLoss/dparams, summed in a fixed order.
  * The adjoint traces every sample once, keeping its vertices on a per-thread
- * tape in device memory (32 B x max_depth per thread of a grid of about 2^21
+ * tape in device memory (32 B x max_depth per thread of a grid of about 2^22
  * threads, held by the context), then walks the path back from the tape;
  * walked vertices need non-zero attenuation. */
 int mcrt_diff_render(mcrt_ctx *ctx, const CameraUniform *camera, uint32_t width,

@@ -19,7 +19,7 @@
 // on the way out.  Instead of re-tracing the path from its PCG stream (which
 // needs no memory), the first trace keeps each vertex's (E_k, att_k,
 // parameter) on a per-thread tape in device memory and the second walk reads
-// it back: 32 B per vertex, 2^21 threads x 6 vertices = 400 MB for the 512^2
+// it back: 32 B per vertex, 2^22 threads x 6 vertices = 805 MB for the 512^2
 // inverse workload, in exchange for half the traversal work.  The values are
 // the ones a re-trace would recompute, so the gradient is unchanged bit for
 // bit.
@@ -27,7 +27,7 @@
 // Layout: one thread per (frame group, pixel), the group's frames looped
 // inside the thread; threads of one wave hold neighbouring pixels of the same
 // group.  A small image is split into several frame groups so that the grid
-// holds about 2^21 threads (several waves per SIMD slot, which balances the
+// holds about 2^22 threads (several waves per SIMD slot, which balances the
 // waves' uneven path lengths).  Gradients sit in registers (kMaxParams x 4),
 // are summed per wave by a fixed butterfly, per block in wave order and then
 // over blocks in block order, so results are reproducible.  The primal sums
@@ -246,10 +246,10 @@ __global__ __launch_bounds__(256) void k_diff_sum(const float *partial, uint32_t
 
 constexpr int kDiffBlock = 256;
 
-// Frame groups per pixel: enough that the grid holds about 2^21 threads,
+// Frame groups per pixel: enough that the grid holds about 2^22 threads,
 // at most one frame each.
 uint32_t diff_groups(uint32_t wh, uint32_t spp) {
-    const uint32_t g = (1u << 21) / wh;
+    const uint32_t g = (1u << 22) / wh;
     return g < 1u ? 1u : (g > spp ? spp : g);
 }
 
