@@ -103,11 +103,15 @@ __device__ __forceinline__ V3 diff_path_radiance(const DiffArgs &a, V3 o, V3 d, 
 
 __device__ __forceinline__ float safe_div(float x, float y) { return y != 0.0f ? x / y : 0.0f; }
 
-// The primal is compiled for 6 waves per SIMD (80 VGPRs, a 12-byte spill;
-// 4% faster per inverse step than the compiler's 5); the adjoint keeps 32
-// gradient accumulators and the compiler's choice.
+// Both kernels are compiled for 8 waves per SIMD, far above the compiler's
+// own choice (primal 5 waves at 89 VGPRs, adjoint 4 at 122): 64 VGPRs, with
+// a 48-byte (primal) and a 192-byte (adjoint: part of the 32 gradient
+// accumulators and traversal state) spill per lane.  More waves hide the
+// latency of dependent node fetches better than the spills cost: the 512^2
+// inverse step went 26.8 -> 21.5 ms, each step up in waves faster than the
+// last.
 template <int kBlock, bool kSpheres, bool kAdjoint>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kAdjoint ? 1 : 6, 8))) void k_diff(DiffArgs a) {
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_diff(DiffArgs a) {
     constexpr int kWaves = kBlock / 64;
     __shared__ uint32_t lds[Stack::kShort * Stack::kWords * kBlock];
     __shared__ float red[kAdjoint ? kMaxParams * 4 * kWaves : 1];
