@@ -134,8 +134,12 @@ struct ExtOccIO {
     }
 };
 
+// Like k_extend, the sphere-free plain-stack variant runs 8 waves per SIMD
+// (64 VGPRs, no spill; the 1080p icosphere frame with next-event estimation:
+// 1 spp 3.98 -> 3.88 ms, 4 spp 13.05 -> 11.96 ms against the compiler's 7).
 template <int kBlock, bool kSpheres, bool kCull>
-__global__ __launch_bounds__(kBlock) void k_extend_occ(ExtArgs a) {
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu((kSpheres || kCull) ? 1 : 8, 8)))
+void k_extend_occ(ExtArgs a) {
     __shared__ uint32_t lds[TStack<kCull>::kShort * TStack<kCull>::kWords * kBlock];
     trace_persistent<kBlock, kSpheres, kCull>(a.scene, ExtOccIO{a, *a.count}, lds, a.ovf, a.ovf_stride);
 }
