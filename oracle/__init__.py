@@ -181,6 +181,16 @@ def gradient_env(w=1024, h=512):
     return out
 
 
+def sample_env(scene: Scene, direction):
+    """Environment radiance the shader looks up for a direction (rgb)."""
+    L = lib()
+    L.oracle_sample_env.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+    d = np.ascontiguousarray(direction, np.float32)
+    out = np.empty(3, np.float32)
+    L.oracle_sample_env(scene.ref(), _p(d), _p(out))
+    return out
+
+
 def pcg_hash(v: int) -> int:
     return int(lib().oracle_pcg_hash(v & 0xFFFFFFFF))
 

@@ -114,6 +114,12 @@ static v3 sample_env(const OracleScene *s, v3 dir) {
     return mk(p[0], p[1], p[2]);
 }
 
+/* Exported for the known-answer tests of the direction -> texel mapping. */
+void oracle_sample_env(const OracleScene *s, const float dir[3], float out[3]) {
+    v3 c = sample_env(s, mk(dir[0], dir[1], dir[2]));
+    out[0] = c.x; out[1] = c.y; out[2] = c.z;
+}
+
 /* Gradient sky, src/texture.rs:158-199. */
 void oracle_gradient_env(uint32_t w, uint32_t h, float *out) {
     for (uint32_t y = 0; y < h; ++y) {

@@ -145,6 +145,8 @@ uint8_t oracle_tonemap_u8(float v);
 
 /* Procedural gradient sky, src/texture.rs:158-199.  out: w*h*4. */
 void oracle_gradient_env(uint32_t w, uint32_t h, float *out);
+/* sample_env (path_trace.wgsl:233-242) for one direction: out = rgb. */
+void oracle_sample_env(const OracleScene *s, const float dir[3], float out[3]);
 
 #ifdef __cplusplus
 }
