@@ -353,3 +353,270 @@ def test_tonemap_closed_form():
     assert oracle.tonemap_u8(float("nan")) == 0 and oracle.tonemap_u8(float("inf")) == 0
     assert oracle.tonemap_u8(1.0) == int(0.5 ** (1 / 2.2) * 255 + 0.5)
     assert oracle.tonemap_u8(1e30) == 255
+
+
+# ---------------------------------------------------------------------------
+# Bilateral weight formula, known answers (denoise.wgsl:65-66, :87-107).
+# An impulse in the accumulation image makes the output at pixel p equal to
+# w(p, q) / sum_t w(p, t), so each term of the weight can be read off.
+# ---------------------------------------------------------------------------
+def _params(radius=3, sigma_s=2.0, sigma_d=0.5, normal_pow=8.0):
+    p = T.default_denoise_params()
+    p["radius"], p["sigma_s"], p["sigma_d"], p["normal_pow"] = radius, sigma_s, sigma_d, normal_pow
+    return p
+
+
+@pytest.mark.parametrize("radius,sigma_s", [(1, 2.0), (3, 2.0), (3, 0.7), (2, 1.3)])
+def test_bilateral_impulse_response_is_the_normalised_gaussian(radius, sigma_s):
+    # Flat G-buffer: every normal and depth term is 1, so the response of an
+    # impulse is the spatial table exp(-r^2 / (2 sigma_s^2)) over its sum.
+    # sigma^2 instead of 2 sigma^2, or a missing normalisation, fails.
+    n = 4 * radius + 5
+    a = np.zeros((n, n, 4), np.float32)
+    c = n // 2
+    a[c, c, :3] = 1.0
+    out = oracle.denoise(a, _gb(n, n), _params(radius, sigma_s))
+    k = np.arange(-radius, radius + 1, dtype=np.float64)
+    table = np.exp(-(k[:, None] ** 2 + k[None, :] ** 2) / (2.0 * sigma_s ** 2))
+    expect = table / table.sum()
+    got = out[c - radius:c + radius + 1, c - radius:c + radius + 1, 0]
+    np.testing.assert_allclose(got, expect, rtol=2e-6, atol=1e-9)
+    # outside the footprint nothing leaks
+    mask = np.ones((n, n), bool)
+    mask[c - radius:c + radius + 1, c - radius:c + radius + 1] = False
+    assert np.all(out[mask][:, 0] == 0.0)
+
+
+def test_bilateral_depth_term_is_relative_to_the_centre_depth():
+    # Two depth regions, 4 | 5, radius 1, sigma_d 0.5.  A neighbour across the
+    # step weighs exp(-((d_p - d_q) / d_p)^2 / (2 sigma_d^2)): seen from depth 4
+    # that is exp(-0.25^2 / 0.5) = exp(-0.125), seen from depth 5
+    # exp(-0.2^2 / 0.5) = exp(-0.08).  An absolute depth difference
+    # (exp(-2)) or division by the neighbour's depth swaps or breaks these.
+    h, w = 9, 10
+    g = _gb(h, w, depth=4.0)
+    g[:, 5:, 3] = 5.0
+    s = 2.0
+    e1, e2 = math.exp(-1 / (2 * s * s)), math.exp(-2 / (2 * s * s))
+    col = e2 + e1 + e2        # a column of three taps at dx = +-1
+    mid = e1 + 1.0 + e1       # the centre column
+    for (src, dst, across) in (((4, 5), (4, 4), math.exp(-0.125)), ((4, 4), (4, 5), math.exp(-0.08))):
+        a = np.zeros((h, w, 4), np.float32)
+        a[src[0], src[1], :3] = 1.0
+        out = oracle.denoise(a, g, _params(1, s, 0.5))
+        wsum = col + mid + across * col
+        expect = e1 * across / wsum
+        assert abs(out[dst[0], dst[1], 0] - expect) < 2e-6 * expect
+
+
+@pytest.mark.parametrize("power", [8.0, 2.5, 1.0])
+def test_bilateral_normal_term_is_clamped_cosine_power(power):
+    # Normals (0,0,1) on the left, (0.6,0,0.8) on the right: cos = 0.8, so a
+    # neighbour across weighs 0.8^p.  A third region facing away (cos < 0)
+    # weighs max(0, cos)^p = 0.
+    h, w = 7, 10
+    g = _gb(h, w)
+    g[:, 5:, :3] = (0.6, 0.0, 0.8)
+    s = 2.0
+    e1, e2 = math.exp(-1 / (2 * s * s)), math.exp(-2 / (2 * s * s))
+    col, mid = e2 + e1 + e2, e1 + 1.0 + e1
+    a = np.zeros((h, w, 4), np.float32)
+    a[3, 5, :3] = 1.0
+    out = oracle.denoise(a, g, _params(1, s, 0.5, power))
+    across = 0.8 ** power
+    expect = e1 * across / (col + mid + across * col)
+    assert abs(out[3, 4, 0] - expect) < 3e-6 * expect
+    g2 = g.copy()
+    g2[:, 5:, :3] = (0.0, 0.6, -0.8)
+    out2 = oracle.denoise(a, g2, _params(1, s, 0.5, power))
+    assert out2[3, 4, 0] == 0.0
+
+
+# ---------------------------------------------------------------------------
+# Environment lookup, known texels (path_trace.wgsl:233-242)
+# u = atan2(d.z, d.x) / tau + 0.5, v = 0.5 - asin(d.y) / pi, texel
+# (u32(u W) % W, u32(v H) % H) of a map whose row 0 is the zenith.
+# ---------------------------------------------------------------------------
+def coded_env():
+    """Env map whose texel (x, y) holds (x, y, 0)."""
+    e = np.zeros((512, 1024, 4), np.float32)
+    e[..., 0] = np.arange(1024, dtype=np.float32)[None, :]
+    e[..., 1] = np.arange(512, dtype=np.float32)[:, None]
+    e[..., 3] = 1.0
+    return e
+
+
+def make_coded_env_scene():
+    loaded = scene_mod.load_scene_from_str("materials:\n  w: {type: lambertian}\n")
+    return oracle.Scene(scene_mod.build_scene_buffers(loaded, env=coded_env()))
+
+
+@pytest.fixture(scope="module")
+def coded_env_scene():
+    return make_coded_env_scene()
+
+
+@pytest.mark.parametrize("d,texel", [
+    ((1, 0, 0), (512, 256)),     # +x: phi 0 -> u 0.5; horizon row
+    ((-1, 0, 0), (0, 256)),      # -x: phi pi -> u 1.0 wraps to column 0
+    ((0, 0, 1), (768, 256)),     # +z: phi pi/2 -> u 0.75
+    ((0, 0, -1), (256, 256)),    # -z: phi -pi/2 -> u 0.25
+    ((0, 1, 0), (512, 0)),       # zenith: row 0
+    ((0, -1, 0), (512, 0)),      # nadir: v = 1.0, u32(512) % 512 wraps to row 0
+])
+def test_env_lookup_axis_directions(coded_env_scene, d, texel):
+    c = oracle.sample_env(coded_env_scene, d)
+    assert (int(c[0]), int(c[1])) == texel
+
+
+def test_env_lookup_oblique_directions(coded_env_scene):
+    rng = np.random.default_rng(4)
+    checked = 0
+    for _ in range(200):
+        d = rng.normal(size=3)
+        d /= np.linalg.norm(d)
+        d = d.astype(np.float32)
+        u = math.atan2(float(d[2]), float(d[0])) / (2 * math.pi) + 0.5
+        v = 0.5 - math.asin(float(d[1])) / math.pi
+        fu, fv = u * 1024, v * 512
+        if min(abs(fu - round(fu)), abs(fv - round(fv))) < 1e-3:
+            continue  # too close to a texel edge for a float64 closed form
+        c = oracle.sample_env(coded_env_scene, d)
+        assert (int(c[0]), int(c[1])) == (int(fu) % 1024, int(fv) % 512)
+        checked += 1
+    assert checked > 150
+    # the (1, 1, 1) diagonal: phi = pi/4 -> column 640; theta = asin(1/sqrt 3) -> row 155
+    c = oracle.sample_env(coded_env_scene, np.full(3, 1 / math.sqrt(3), np.float32))
+    assert (int(c[0]), int(c[1])) == (640, 155)
+
+
+# ---------------------------------------------------------------------------
+# Camera draw prefix (path_trace.wgsl:636-651): 2 jitter draws (x, then y,
+# with v flipped so row 0 is the top), 2 lens draws even for a pinhole, then
+# the first scatter.
+# ---------------------------------------------------------------------------
+def _pcg_stream(seed, k):
+    s, out = seed, []
+    for _ in range(k):
+        r, s = oracle.rand_f32(s)
+        out.append(r)
+    return out
+
+
+def _camera(origin, lower_left, horizontal, vertical, du=(0, 0, 0), dv=(0, 0, 0)):
+    c = np.zeros((), T.CameraUniform)
+    c["origin"][:3], c["lower_left"][:3] = origin, lower_left
+    c["horizontal"][:3], c["vertical"][:3] = horizontal, vertical
+    c["defocus_u"][:3], c["defocus_v"][:3] = du, dv
+    return c
+
+
+def _tilted_plane_scene(env=None):
+    # a large plane through (0, 0, -5) tilted about x: z = -5 + 0.5 y
+    n = np.array([0.0, -0.5, 1.0])
+    n /= np.linalg.norm(n)
+    text = ("materials:\n  w: {type: lambertian, albedo: [1, 1, 1]}\nobjects:\n"
+            f"  - {{type: plane, normal: [{n[0]}, {n[1]}, {n[2]}], half_extents: [50, 50], material: w,"
+            " transform: {translate: [0, 0, -5]}}\n")
+    loaded = scene_mod.load_scene_from_str(text)
+    return oracle.Scene(scene_mod.build_scene_buffers(loaded, env=env)), n
+
+
+def _plane_t(o, d, n):
+    # plane n . (p - (0, 0, -5)) = 0
+    p0 = np.array([0.0, 0.0, -5.0])
+    return float(np.dot(n, p0 - o) / np.dot(n, d))
+
+
+def test_camera_jitter_draws_fix_the_first_hit_depth():
+    sc, n = _tilted_plane_scene()
+    W, H = 256, 128
+    cam = _camera((0, 0, 0), (-2, -1, -1), (4, 0, 0), (0, 2, 0))
+    for (x, y, frame) in ((0, 0, 0), (37, 101, 5), (255, 127, 2)):
+        r1, r2 = _pcg_stream(oracle.pixel_seed(x, y, W, frame), 2)
+        u = (x + r1) / W
+        v = 1.0 - (y + r2) / H
+        d = np.array([-2 + 4 * u, -1 + 2 * v, -1.0])
+        d /= np.linalg.norm(d)
+        t = _plane_t(np.zeros(3), d, n)
+        _, gb = oracle.sample(sc, cam, W, H, x, y, frame, max_depth=1)
+        assert abs(gb[3] - t) < 2e-6 * t, (x, y, frame, gb[3], t)
+    assert oracle.pixel_seed(0, 0, 256, 0) == 817759070  # the seed of pixel (0, 0) above
+
+
+def test_camera_lens_draws_offset_the_origin():
+    sc, n = _tilted_plane_scene()
+    W, H = 256, 128
+    du, dv = (0.3, 0.0, 0.0), (0.0, 0.2, 0.0)
+    cam = _camera((0, 0, 0), (-2, -1, -1), (4, 0, 0), (0, 2, 0), du, dv)
+    for (x, y, frame) in ((0, 0, 0), (200, 40, 9)):
+        r1, r2, r3, r4 = _pcg_stream(oracle.pixel_seed(x, y, W, frame), 4)
+        u = (x + r1) / W
+        v = 1.0 - (y + r2) / H
+        focal = np.array([-2 + 4 * u, -1 + 2 * v, -1.0])
+        rr, th = math.sqrt(r3), 2 * math.pi * r4
+        o = rr * math.cos(th) * np.array(du) + rr * math.sin(th) * np.array(dv)
+        d = focal - o
+        d /= np.linalg.norm(d)
+        t = _plane_t(o, d, n)
+        _, gb = oracle.sample(sc, cam, W, H, x, y, frame, max_depth=1)
+        assert abs(gb[3] - t) < 2e-6 * t, (x, y, frame, gb[3], t)
+
+
+def test_first_scatter_uses_draws_five_and_six():
+    # A white Lambertian plane under a coordinate-coded sky: the sample is the
+    # texel its cosine-scattered ray reaches, so the draws the scatter used can
+    # be read back.  Draws 1-2 jitter, 3-4 lens (pinhole), 5-6 the scatter.
+    e = coded_env()
+    text = ("materials:\n  w: {type: lambertian, albedo: [1, 1, 1]}\nobjects:\n"
+            "  - {type: plane, normal: [0, 1, 0], half_extents: [1000, 1000], material: w}\n")
+    loaded = scene_mod.load_scene_from_str(text)
+    sc = oracle.Scene(scene_mod.build_scene_buffers(loaded, env=e))
+    W, H = 64, 64
+    cam = _camera((0, 2, 0), (-1, 1, -1), (2, 0, 0), (0, -2, 0))  # looking at the floor
+    checked = 0
+    for x, y in ((3, 5), (40, 22), (61, 60), (17, 47)):
+        draws = _pcg_stream(oracle.pixel_seed(x, y, W, 0), 6)
+        cos_phi = 1.0 - 2.0 * draws[4]
+        sin_phi = math.sqrt(max(0.0, 1.0 - cos_phi * cos_phi))
+        theta = 2 * math.pi * draws[5]
+        s = np.array([0.0, 1.0, 0.0]) + np.array([sin_phi * math.cos(theta), sin_phi * math.sin(theta), cos_phi])
+        s /= np.linalg.norm(s)
+        u = math.atan2(s[2], s[0]) / (2 * math.pi) + 0.5
+        v = 0.5 - math.asin(s[1]) / math.pi
+        fu, fv = u * 1024, v * 512
+        if min(abs(fu - round(fu)), abs(fv - round(fv))) < 1e-2:
+            continue
+        rgb, gb = oracle.sample(sc, cam, W, H, x, y, 0, max_depth=8)
+        assert gb[3] < 1e8
+        assert (int(rgb[0]), int(rgb[1])) == (int(fu) % 1024, int(fv) % 512), (x, y)
+        checked += 1
+    assert checked >= 3
+
+
+# ---------------------------------------------------------------------------
+# Running mean and its freeze at MAX_FRAME_COUNT = 65535 (:58, :657-659)
+# ---------------------------------------------------------------------------
+def test_running_mean_weight_freezes_at_65535():
+    w = configs.c1(width=8, height=8)
+    sc = oracle.Scene(w.buffers)
+    prev = np.full((8, 8, 4), 0.3125, np.float32)
+    prev[..., 3] = 1
+    for frame, weight in ((0, 1.0), (1, 0.5), (65534, 1 / 65535), (65535, 1 / 65536), (65536, 1 / 65536),
+                          (1 << 24, 1 / 65536)):
+        acc, _ = oracle.render(sc, w.camera, 8, 8, 1, first_frame=frame, accum_in=prev, threads=2)
+        a = np.float32(np.float32(1.0) / np.float32(weight))  # min(f, 65535) + 1
+        wf = np.float32(1.0) / a
+        for (x, y) in ((1, 2), (6, 5), (3, 3)):
+            s, _ = oracle.sample(sc, w.camera, 8, 8, x, y, frame, 8)
+            # mix(prev, s, w) = prev * (1 - w) + s * w, each step rounded to f32
+            expect = prev[y, x, :3] * (np.float32(1.0) - wf) + s * wf
+            assert np.array_equal(acc[y, x, :3], expect.astype(np.float32)), (frame, x, y)
+        # independent of the mix form: from a zero running mean the result is
+        # the sample times the weight, exact for the power-of-two weights
+        acc0, _ = oracle.render(sc, w.camera, 8, 8, 1, first_frame=frame, accum_in=np.zeros_like(prev),
+                                threads=2)
+        if frame >= 65535 or frame <= 1:
+            for (x, y) in ((1, 2), (6, 5)):
+                s, _ = oracle.sample(sc, w.camera, 8, 8, x, y, frame, 8)
+                assert np.array_equal(acc0[y, x, :3], (s.astype(np.float64) * weight).astype(np.float32))
