@@ -146,7 +146,8 @@ int mcrt_create(mcrt_ctx **out_ctx, int device);
 void mcrt_destroy(mcrt_ctx *ctx);
 
 /* Message of the last failed call on `ctx` ("" if none).  With ctx == NULL,
- * the message of the last failed mcrt_create on this thread. */
+ * the message of the last failed mcrt_create or context-free call
+ * (mcrt_validate_bvh) on this thread. */
 const char *mcrt_last_error(const mcrt_ctx *ctx);
 
 /* Replaces gpu_layout::build_scene_buffers (src/gpu_layout.rs:259-390) after
@@ -260,6 +261,16 @@ int mcrt_build_mesh_bvh(const GpuVertex *verts, uint32_t n_verts,
                         const GpuTriangle *tris, uint32_t n_tris,
                         GpuBvhNode *out_nodes, uint32_t *out_n_nodes,
                         GpuTriangle *out_ordered, uint32_t *out_orig_index);
+
+/* Checks a reference pre-order BVH (src/bvh.rs:54-68 layout, as produced by
+ * build_mesh_bvh / build_bvh) the way mcrt_upload_scene does before it
+ * converts one: children inside the pre-order layout, every node reached from
+ * the root exactly once, leaves within [0, n_prims) and disjoint, every
+ * primitive in some leaf.  The reference's empty forms (n_nodes = 0, or one
+ * zeroed node) pass.  Host-only; MCRT_ERR_BAD_ARG with the reason in
+ * mcrt_last_error(NULL).  No counterpart call in the reference, whose host
+ * builds its trees itself (src/gpu_layout.rs:259-340 uploads them unchecked). */
+int mcrt_validate_bvh(const GpuBvhNode *nodes, uint32_t n_nodes, uint32_t n_prims);
 
 int mcrt_build_sphere_bvh(const GpuSphere *spheres, uint32_t n_spheres,
                           GpuBvhNode *out_nodes, uint32_t *out_n_nodes,

@@ -76,6 +76,7 @@ def load() -> C.CDLL:
         "mcrt_trace_closest_dev": (C.c_int, [_vp, _vp, _vp, _u32, _f32, _f32, _vp, _vp, _vp]),
         "mcrt_build_mesh_bvh": (C.c_int, [_vp, _u32, _vp, _u32, _vp, C.POINTER(_u32), _vp, _vp]),
         "mcrt_build_sphere_bvh": (C.c_int, [_vp, _u32, _vp, C.POINTER(_u32), _vp, _vp]),
+        "mcrt_validate_bvh": (C.c_int, [_vp, _u32, _u32]),
         "mcrt_set_profiling": (C.c_int, [_vp, C.c_int]),
         "mcrt_set_nee": (C.c_int, [_vp, C.c_int]),
         "mcrt_kernel_stats": (C.c_int, [_vp, C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_uint64)]),
@@ -115,13 +116,23 @@ def exported_symbols() -> list:
         "mcrt_create", "mcrt_destroy", "mcrt_last_error", "mcrt_upload_scene", "mcrt_render",
         "mcrt_render_dev", "mcrt_denoise_bilateral", "mcrt_denoise_bilateral_dev",
         "mcrt_trace_closest", "mcrt_trace_closest_dev", "mcrt_build_mesh_bvh",
-        "mcrt_build_sphere_bvh", "mcrt_set_profiling", "mcrt_set_nee", "mcrt_kernel_stats", "mcrt_reset_stats",
+        "mcrt_build_sphere_bvh", "mcrt_validate_bvh", "mcrt_set_profiling", "mcrt_set_nee", "mcrt_kernel_stats", "mcrt_reset_stats",
         "mcrt_render_counters", "mcrt_mesh_voxel_chunks", "mcrt_diff_render", "mcrt_diff_render_dev",
         "mcrt_diff_adjoint", "mcrt_diff_adjoint_dev", "mcrt_render_aux_dev", "mcrt_denoiser_create",
         "mcrt_denoiser_destroy", "mcrt_denoiser_param_count", "mcrt_denoiser_get_weights",
         "mcrt_denoiser_set_weights", "mcrt_denoiser_infer_dev", "mcrt_denoiser_train_step_dev",
         "mcrt_denoiser_gradient", "mcrt_denoiser_set_loss",
     ]
+
+
+def validate_bvh(nodes: np.ndarray, n_prims: int) -> None:
+    """mcrt_validate_bvh: raises McrtError with the library's reason for a
+    malformed reference pre-order tree."""
+    lib = load()
+    nodes = np.ascontiguousarray(nodes, T.GpuBvhNode)
+    rc = lib.mcrt_validate_bvh(_ptr(nodes) if len(nodes) else None, len(nodes), n_prims)
+    if rc != MCRT_OK:
+        raise McrtError(rc, lib.mcrt_last_error(None).decode())
 
 
 def mesh_voxel_chunks(blocks: np.ndarray, cx0: int, cz0: int, voxel_size: float,

@@ -86,7 +86,10 @@ int fail(mcrt_ctx *c, int code, const char *fmt, ...) {
     va_start(ap, fmt);
     vsnprintf(buf, sizeof buf, fmt, ap);
     va_end(ap);
-    if (c) c->err = buf;
+    if (c)
+        c->err = buf;
+    else
+        g_create_error = buf;  // context-free calls (mcrt_validate_bvh)
     return code;
 }
 
@@ -219,6 +222,30 @@ int convert_tree(mcrt_ctx *c, const char *what, const GpuBvhNode *nodes, uint32_
             out.last[end - 1] = 1;
         }
     }
+    // Tree shape: every node is reached from the root exactly once (a node
+    // with two parents, or one no parent reaches, is rejected before any
+    // index is derived from it), and every primitive lies in a reached leaf.
+    {
+        std::vector<uint8_t> seen(n, 0);
+        std::vector<uint32_t> todo{0u};
+        while (!todo.empty()) {
+            const uint32_t i = todo.back();
+            todo.pop_back();
+            if (seen[i])
+                return fail(c, MCRT_ERR_BAD_ARG, "%s BVH: node %u is reached from two parents", what, i);
+            seen[i] = 1;
+            if (nodes[i].prim_count == 0) {
+                todo.push_back(nodes[i].right_or_offset);
+                todo.push_back(i + 1);
+            }
+        }
+        for (uint32_t i = 0; i < n; ++i)
+            if (!seen[i])
+                return fail(c, MCRT_ERR_BAD_ARG, "%s BVH: node %u is not reachable from the root", what, i);
+        for (uint32_t k = 0; k < n_prims; ++k)
+            if (!covered[k])
+                return fail(c, MCRT_ERR_BAD_ARG, "%s BVH: primitive %u is in no leaf", what, k);
+    }
     // Device order: the internal children of a node take consecutive indices,
     // a pair starting at an even index, so siblings share one 128-B line (one
     // 64-B node each) and a traversal that visits both fetches the line once.
@@ -289,6 +316,15 @@ int convert_tree(mcrt_ctx *c, const char *what, const GpuBvhNode *nodes, uint32_
     out.depth = std::max<uint32_t>(maxd, 1);
     return MCRT_OK;
 }
+
+}  // namespace
+
+extern "C" int mcrt_validate_bvh(const GpuBvhNode *nodes, uint32_t n_nodes, uint32_t n_prims) {
+    TreeOut out;
+    return convert_tree(nullptr, "given", nodes, n_nodes, n_prims, out);
+}
+
+namespace {
 
 inline float ubits(uint32_t u) {
     float f;

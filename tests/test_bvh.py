@@ -206,3 +206,57 @@ def test_sphere_bvh_structure():
     c = ordered["center_r"]
     _check_structure(nodes, 200, (c[:, :3] - c[:, 3:], c[:, :3] + c[:, 3:]))
     assert np.array_equal(ordered, s[orig])
+
+
+# ---------------------------------------------------------------------------
+# Tree validation shared with mcrt_upload_scene (mcrt_validate_bvh)
+# ---------------------------------------------------------------------------
+def _tree(spec):
+    """spec: per node ('i', right) for internal or ('l', first, count)."""
+    nodes = np.zeros(len(spec), T.GpuBvhNode)
+    for i, s in enumerate(spec):
+        if s[0] == "i":
+            nodes[i]["right_or_offset"], nodes[i]["prim_count"] = s[1], 0
+        else:
+            nodes[i]["right_or_offset"], nodes[i]["prim_count"] = s[1], s[2]
+        nodes[i]["aabb_min"][:3], nodes[i]["aabb_max"][:3] = -1, 1
+    return nodes
+
+
+def test_validate_accepts_builder_trees_and_stubs():
+    from mcrt_amd import _lib
+
+    v, t, _ = scenes.displaced_icosphere(n=6, seed=1, n_mats=1)
+    nodes, _, _ = build_mesh_bvh(v, t)
+    _lib.validate_bvh(nodes, len(t))
+    _lib.validate_bvh(np.zeros(0, T.GpuBvhNode), 0)           # empty
+    _lib.validate_bvh(np.zeros(1, T.GpuBvhNode), 0)           # the reference's zeroed stub node
+    _lib.validate_bvh(_tree([("l", 0, 3)]), 3)                 # a single leaf root
+    _lib.validate_bvh(_tree([("i", 2), ("l", 0, 2), ("l", 2, 1)]), 3)
+
+
+@pytest.mark.parametrize("spec,n_prims,reason", [
+    # node 2 is internal but no parent reaches it (the case that used to
+    # index past the converted-node vector)
+    ([("i", 3), ("l", 0, 1), ("i", 4), ("l", 1, 1), ("l", 2, 1)], 3, "not reachable"),
+    # node 3 is the right child of both 0 and 1
+    ([("i", 3), ("i", 3), ("l", 0, 1), ("l", 1, 1)], 2, "two parents"),
+    # leaves overlap
+    ([("i", 2), ("l", 0, 2), ("l", 1, 2)], 3, "two leaves"),
+    # primitive 3 in no leaf
+    ([("i", 2), ("l", 0, 2), ("l", 2, 1)], 4, "in no leaf"),
+    # leaf beyond the primitive count
+    ([("i", 2), ("l", 0, 2), ("l", 2, 5)], 4, "beyond"),
+    # right child outside the layout / not after the left child
+    ([("i", 7), ("l", 0, 1), ("l", 1, 1)], 2, "outside the pre-order"),
+    ([("i", 1), ("l", 0, 1), ("l", 1, 1)], 2, "outside the pre-order"),
+    # an internal node as the last node has no left child
+    ([("i", 2), ("l", 0, 1), ("i", 3)], 1, "outside the pre-order"),
+])
+def test_validate_rejects_malformed_trees(spec, n_prims, reason):
+    from mcrt_amd import _lib
+
+    with pytest.raises(_lib.McrtError) as e:
+        _lib.validate_bvh(_tree(spec), n_prims)
+    assert e.value.code == -2  # MCRT_ERR_BAD_ARG
+    assert reason in str(e.value), str(e.value)
