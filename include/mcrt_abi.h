@@ -224,7 +224,11 @@ int mcrt_render_aux_dev(mcrt_ctx *ctx, const CameraUniform *camera,
 
 /* Replaces cs_denoise (src/shaders/denoise.wgsl:52-120) as dispatched by
  * src/gpu.rs:1150-1169.  accum/gbuf/out: W*H RGBA32F.  params == NULL uses
- * the reference defaults (src/gpu.rs:177-183). */
+ * the reference defaults (src/gpu.rs:177-183).  Any radius up to 256 is
+ * filtered; a negative radius copies every pixel (the shader's loops run no
+ * tap and its weight sum stays 0).  Radii above 256 ((2R+1)^2 > 263k taps
+ * per pixel) return MCRT_ERR_LIMIT.  The _dev call uploads nothing and never
+ * synchronises: the spatial weights are formed in the kernel. */
 int mcrt_denoise_bilateral(mcrt_ctx *ctx, const DenoiseParams *params,
                            uint32_t width, uint32_t height,
                            const float *accum, const float *gbuf, float *out);

@@ -63,10 +63,7 @@ struct mcrt_ctx {
     size_t cap_counters = 0;
     DevBuf host_accum, host_gbuf, host_out, host_accum_in;
     DevBuf trace_o, trace_d, trace_id, trace_t, trace_work;
-    DevBuf spatial;
     DevBuf diff_partial, diff_grad, diff_image, diff_tape;
-    int spatial_radius = -1;
-    float spatial_sigma = 0.0f;
     // last render bookkeeping
     uint32_t last_batches = 0, last_depth = 0, last_halves = 1;
     uint64_t last_samples = 0;
@@ -415,7 +412,7 @@ extern "C" void mcrt_destroy(mcrt_ctx *c) {
                       &c->env, &c->A[0], &c->A[1], &c->B[0], &c->B[1], &c->C[0], &c->C[1], &c->H,
                       &c->samples, &c->counters, &c->host_accum, &c->host_gbuf, &c->host_out,
                       &c->host_accum_in, &c->trace_o, &c->trace_d, &c->trace_id, &c->trace_t,
-                      &c->trace_work, &c->spatial, &c->diff_partial, &c->diff_grad, &c->diff_image, &c->diff_tape, &c->emit,
+                      &c->trace_work, &c->diff_partial, &c->diff_grad, &c->diff_image, &c->diff_tape, &c->emit,
                       &c->emit_cdf, &c->tri_emit, &c->sph_emit, &c->P[0], &c->P[1], &c->S_o, &c->S_d,
                       &c->S_c})
         release(*b);
@@ -1027,37 +1024,25 @@ extern "C" int mcrt_denoise_bilateral_dev(mcrt_ctx *c, const DenoiseParams *para
     DenoiseParams p = params ? *params : DenoiseParams{3, 2.0f, 0.5f, 8.0f};
     if (width == 0 || height == 0) return MCRT_OK;
     if (!accum_dev || !gbuf_dev || !out_dev) return fail(c, MCRT_ERR_BAD_ARG, "null image pointer");
-    if (p.radius < 0 || p.radius > 8)
-        return fail(c, MCRT_ERR_LIMIT, "denoise radius %d outside the supported range [0, 8]", p.radius);
+    if (p.radius > kMaxBilateralRadius)
+        return fail(c, MCRT_ERR_LIMIT,
+                    "denoise radius %d exceeds %d (a (2R+1)^2 = %lld-tap footprint per pixel)", p.radius,
+                    kMaxBilateralRadius, (long long)(2 * kMaxBilateralRadius + 1) * (2 * kMaxBilateralRadius + 1));
     if (width > 0x7FFFFFFFu || height > 0x7FFFFFFFu) return fail(c, MCRT_ERR_LIMIT, "image too large");
     hipStream_t s = (hipStream_t)stream;
     c->last_stream = s;
-    // Spatial weights exp(-dist_sq * inv2ss), denoise.wgsl:65 and :88-89,
-    // uploaded once per (radius, sigma_s).  The table's device copy is
-    // rewritten only after the streams that may still read it have drained.
-    const int R = p.radius, span = 2 * R + 1;
-    if (!c->spatial.p || c->spatial_radius != R || c->spatial_sigma != p.sigma_s) {
-        HIP_TRY(c, hipDeviceSynchronize());
-        const float inv2ss = 1.0f / (2.0f * p.sigma_s * p.sigma_s);
-        std::vector<float> lut((size_t)span * span);
-        for (int dy = -R, k = 0; dy <= R; ++dy)
-            for (int dx = -R; dx <= R; ++dx, ++k)
-                lut[k] = std::exp(-(float)(dx * dx + dy * dy) * inv2ss);
-        if ((rc = ensure(c, c->spatial, lut.size() * 4))) return rc;
-        HIP_TRY(c, hipMemcpy(c->spatial.p, lut.data(), lut.size() * 4, hipMemcpyHostToDevice));
-        c->spatial_radius = R;
-        c->spatial_sigma = p.sigma_s;
-    }
+    // inv2ss / inv2ds as denoise.wgsl:64-65; the spatial weights are formed
+    // in the kernel, so nothing is uploaded and the call is stream-ordered.
     BilateralArgs a{};
     a.accum = (const float4 *)accum_dev;
     a.gbuf = (const float4 *)gbuf_dev;
     a.out = (float4 *)out_dev;
     a.width = (int32_t)width;
     a.height = (int32_t)height;
-    a.radius = R;
+    a.radius = p.radius;
+    a.inv2ss = 1.0f / (2.0f * p.sigma_s * p.sigma_s);
     a.inv2ds = 1.0f / (2.0f * p.sigma_d * p.sigma_d);
     a.normal_pow = p.normal_pow;
-    a.spatial = (const float *)c->spatial.p;
     {
         Timed t(c, "bilateral", s);
         HIP_TRY(c, launch_bilateral(a, s));

@@ -117,10 +117,13 @@ struct BilateralArgs {
     const float4 *accum, *gbuf;
     float4 *out;
     int32_t width, height;
-    int32_t radius;
-    float inv2ds, normal_pow;
-    const float *spatial;  // (2R+1)^2 weights exp(-r^2/(2 sigma_s^2)), row-major
+    int32_t radius;        // any value up to kMaxBilateralRadius; negative = no tap
+    float inv2ss, inv2ds, normal_pow;
 };
+
+// Largest radius the bilateral launcher accepts: a 513 x 513 footprint,
+// 263k taps per pixel.
+constexpr int32_t kMaxBilateralRadius = 256;
 
 constexpr int kMaxParams = 8;
 

@@ -238,14 +238,50 @@ def test_bilateral_matches_oracle(ctx):
     acc, gb = ctx.render(w.camera, 128, 96, 4, 0, 8)
     # Whole exponents take the kernel's repeated-squaring path (0 included:
     # pow(x, 0) = 1 even at x = 0); 7.5 and 0.3 take the general pow.
+    # Radii 8 and below are staged in LDS; 9, 12 and 20 read taps from memory;
+    # a negative radius runs no tap and copies the pixel, as the shader's
+    # empty loops do.
     for radius, ss, sd, npow in ((3, 2.0, 0.5, 8.0), (1, 1.0, 0.25, 2.0), (5, 3.0, 1.0, 16.0), (0, 2.0, 0.5, 8.0),
-                                 (2, 1.5, 0.75, 7.5), (3, 2.0, 0.5, 0.3), (3, 2.0, 0.5, 0.0)):
+                                 (2, 1.5, 0.75, 7.5), (3, 2.0, 0.5, 0.3), (3, 2.0, 0.5, 0.0), (8, 4.0, 0.5, 8.0),
+                                 (9, 4.0, 0.5, 8.0), (12, 5.0, 0.4, 8.0), (20, 3.0, 0.5, 2.5), (-1, 2.0, 0.5, 8.0),
+                                 (-7, 2.0, 0.5, 8.0)):
         p = T.default_denoise_params()
         p["radius"], p["sigma_s"], p["sigma_d"], p["normal_pow"] = radius, ss, sd, npow
         g = ctx.denoise_bilateral(acc, gb, p)
         o = oracle.denoise(acc, gb, p)
         np.testing.assert_allclose(g[..., :3], o[..., :3], rtol=2e-5, atol=1e-6)
         assert np.all(g[..., 3] == 1.0)
+
+
+def test_bilateral_radius_limit_and_stream_order(ctx):
+    """Radii above 256 are refused with a message; parameter changes between
+    calls need no device-wide synchronisation (each call is stream-ordered)."""
+    import torch
+
+    acc = np.random.default_rng(3).random((40, 48, 4)).astype(np.float32)
+    gb = np.zeros((40, 48, 4), np.float32)
+    gb[..., 2] = 1.0
+    gb[..., 3] = 5.0
+    p = T.default_denoise_params()
+    p["radius"] = 257
+    with pytest.raises(Exception) as e:
+        ctx.denoise_bilateral(acc, gb, p)
+    assert "256" in str(e.value)
+    dev = torch.device("cuda:0")
+    ta, tg = torch.from_numpy(acc).to(dev), torch.from_numpy(gb).to(dev)
+    outs = [torch.empty_like(ta) for _ in range(3)]
+    s = torch.cuda.Stream(dev)
+    with torch.cuda.stream(s):
+        for o, (r, sg) in zip(outs, ((3, 2.0), (1, 0.5), (10, 3.0))):
+            q = T.default_denoise_params()
+            q["radius"], q["sigma_s"] = r, sg
+            ctx.denoise_bilateral_dev(q, 48, 40, ta.data_ptr(), tg.data_ptr(), o.data_ptr(), s.cuda_stream)
+    s.synchronize()
+    for o, (r, sg) in zip(outs, ((3, 2.0), (1, 0.5), (10, 3.0))):
+        q = T.default_denoise_params()
+        q["radius"], q["sigma_s"] = r, sg
+        np.testing.assert_allclose(o.cpu().numpy()[..., :3], oracle.denoise(acc, gb, q)[..., :3],
+                                   rtol=2e-5, atol=1e-6)
 
 
 def test_bilateral_sky_passthrough_quirk(ctx):
