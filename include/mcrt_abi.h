@@ -290,10 +290,8 @@ int mcrt_build_sphere_bvh(const GpuSphere *spheres, uint32_t n_spheres,
  *  - image: W*H RGBA32F, the mean of frames first_frame..first_frame+spp-1.
  *  - d_image: W*H RGBA32F, dLoss/dimage (alpha ignored).
  *  - grad: n_params x 4 floats, dLoss/dparams, summed in a fixed order.
- * The adjoint traces every sample once, keeping its vertices on a per-thread
- * tape in device memory (32 B x max_depth per thread of a grid of about 2^22
- * threads, held by the context), then walks the path back from the tape;
- * walked vertices need non-zero attenuation. */
+ * The adjoint re-traces every sample's path from its PCG stream (path
+ * replay) and needs no memory beyond its output; see mcrt_diff_set_mode. */
 int mcrt_diff_render(mcrt_ctx *ctx, const CameraUniform *camera, uint32_t width,
                      uint32_t height, uint32_t spp, uint32_t first_frame,
                      uint32_t max_depth, const float *params, uint32_t n_params,
@@ -313,6 +311,17 @@ int mcrt_diff_adjoint_dev(mcrt_ctx *ctx, const CameraUniform *camera, uint32_t w
                           uint32_t max_depth, const float *params, uint32_t n_params,
                           const int32_t *param_of_material, const float *d_image_dev,
                           float *out_grad_dev, void *stream);
+
+/* Adjoint form of this context's mcrt_diff_adjoint* calls:
+ *  - MCRT_DIFF_REPLAY (default): one re-trace per sample, parameter
+ *    derivatives carried along the path; no tape, no division by albedos.
+ *  - MCRT_DIFF_TAPE: the trace stores each vertex (48 B x max_depth per
+ *    thread of a grid of about 2^22 threads, held by the context), and the
+ *    gradient is summed walking the tape backwards.
+ * Both are exact for zero albedos and agree to FP32 rounding. */
+#define MCRT_DIFF_REPLAY 0
+#define MCRT_DIFF_TAPE 1
+int mcrt_diff_set_mode(mcrt_ctx *ctx, int mode);
 
 /* Learned denoiser.  No counterpart in the reference (its only denoiser is
  * the bilateral filter, which stays the fallback): six 3x3 convolutions

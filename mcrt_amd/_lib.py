@@ -87,6 +87,7 @@ def load() -> C.CDLL:
         "mcrt_diff_adjoint": (C.c_int, [_vp, _vp, _u32, _u32, _u32, _u32, _u32, _vp, _u32, _vp, _vp, _vp]),
         "mcrt_diff_adjoint_dev": (C.c_int, [_vp, _vp, _u32, _u32, _u32, _u32, _u32, _vp, _u32, _vp, _vp, _vp,
                                             _vp]),
+        "mcrt_diff_set_mode": (C.c_int, [_vp, C.c_int]),
         "mcrt_render_aux_dev": (C.c_int, [_vp, _vp, _u32, _u32, _u32, _u32, _u32, _vp, _vp, _vp, _vp, _vp]),
         "mcrt_denoiser_create": (C.c_int, [_vp, _u32, C.POINTER(_vp)]),
         "mcrt_denoiser_destroy": (None, [_vp]),
@@ -118,7 +119,7 @@ def exported_symbols() -> list:
         "mcrt_trace_closest", "mcrt_trace_closest_dev", "mcrt_build_mesh_bvh",
         "mcrt_build_sphere_bvh", "mcrt_validate_bvh", "mcrt_set_profiling", "mcrt_set_nee", "mcrt_kernel_stats", "mcrt_reset_stats",
         "mcrt_render_counters", "mcrt_mesh_voxel_chunks", "mcrt_diff_render", "mcrt_diff_render_dev",
-        "mcrt_diff_adjoint", "mcrt_diff_adjoint_dev", "mcrt_render_aux_dev", "mcrt_denoiser_create",
+        "mcrt_diff_adjoint", "mcrt_diff_adjoint_dev", "mcrt_diff_set_mode", "mcrt_render_aux_dev", "mcrt_denoiser_create",
         "mcrt_denoiser_destroy", "mcrt_denoiser_param_count", "mcrt_denoiser_get_weights",
         "mcrt_denoiser_set_weights", "mcrt_denoiser_infer_dev", "mcrt_denoiser_train_step_dev",
         "mcrt_denoiser_gradient", "mcrt_denoiser_set_loss",
@@ -350,6 +351,10 @@ class Context:
 
     def set_profiling(self, enabled: bool) -> None:
         self._check(self._lib.mcrt_set_profiling(self._h, 1 if enabled else 0))
+
+    def set_diff_mode(self, mode: str) -> None:
+        """Adjoint form: "replay" (default, no tape) or "tape"."""
+        self._check(self._lib.mcrt_diff_set_mode(self._h, {"replay": 0, "tape": 1}[mode]))
 
     def set_nee(self, enabled: bool) -> None:
         """Next-event estimation with MIS for later renders (default off)."""

@@ -13,16 +13,28 @@
 //   dL/dE_p    = sum over vertices on p of T_k                 (white)
 //   dL/da_p[c] = sum over vertices on p of T_k[c] * R_{k+1}[c].
 //
-// Path replay (after Vicini, Speierer, Jakob, "Path Replay Backpropagation",
-// ACM TOG 40(4), 2021): the adjoint kernel traces each sample once to get L,
-// then walks the same path again, recovering R_{k+1} = (R_k - E_k) / att_k
-// on the way out.  Instead of re-tracing the path from its PCG stream (which
-// needs no memory), the first trace keeps each vertex's (E_k, att_k,
-// parameter) on a per-thread tape in device memory and the second walk reads
-// it back: 32 B per vertex, 2^22 threads x 6 vertices = 805 MB for the 512^2
-// inverse workload, in exchange for half the traversal work.  The values are
-// the ones a re-trace would recompute, so the gradient is unchanged bit for
-// bit.
+// Two adjoint forms, selected per context (mcrt_diff_set_mode):
+//
+// Replay (default; after Vicini, Speierer, Jakob, "Path Replay
+// Backpropagation", ACM TOG 40(4), 2021): the adjoint re-traces each sample
+// from its PCG stream, exactly once, and keeps no tape.  Where PRB walks
+// forward with the remaining radiance L - sum_{j<=k} T_j E_j and divides by
+// the BSDF value at each vertex (ill-conditioned at small albedos, undefined
+// at zero), this estimator has at most kMaxParams parameters, so the replay
+// carries their derivatives forward instead: D_p = dT/da_p, times the pixel's
+// adjoint delta, updated per vertex as
+//   g_p    += E_k * D_p            (every p; E_k the radiance added at k)
+//   g_p.w  += delta . T_k          (p = parameter of vertex k)
+//   D_p     = D_p * att_k  (+ delta * T_k for p = parameter of vertex k)
+// which sums exactly the terms T_k * R_{k+1} above, with no division and
+// without the primal radiance L.
+//
+// Tape: the trace writes each vertex's (E_k, parameter), (att_k, continues)
+// and T_k to a per-thread tape in device memory (48 B per vertex, about 1.2
+// GB for the 512^2 inverse workload), then walks it backwards,
+// R = E_k + att_k * R, adding delta * T_k * R_{k+1} at parameter vertices:
+// also division-free.  Same terms, other summation order: the two forms
+// agree to FP32 rounding.
 //
 // Layout: one thread per (frame group, pixel), the group's frames looped
 // inside the thread; threads of one wave hold neighbouring pixels of the same
@@ -101,8 +113,6 @@ __device__ __forceinline__ V3 diff_path_radiance(const DiffArgs &a, V3 o, V3 d, 
     return L;
 }
 
-__device__ __forceinline__ float safe_div(float x, float y) { return y != 0.0f ? x / y : 0.0f; }
-
 // Both kernels are compiled for 8 waves per SIMD, far above the compiler's
 // own choice (primal 5 waves at 89 VGPRs, adjoint 4 at 122): 64 VGPRs, with
 // a 48-byte (primal) and a 192-byte (adjoint: part of the 32 gradient
@@ -110,8 +120,10 @@ __device__ __forceinline__ float safe_div(float x, float y) { return y != 0.0f ?
 // latency of dependent node fetches better than the spills cost: the 512^2
 // inverse step went 26.8 -> 21.5 ms, each step up in waves faster than the
 // last.
-template <int kBlock, bool kSpheres, bool kAdjoint>
+// kMode: 0 primal, 1 replay adjoint, 2 tape adjoint.
+template <int kBlock, bool kSpheres, int kMode>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_diff(DiffArgs a) {
+    constexpr bool kAdjoint = kMode != 0;
     constexpr int kWaves = kBlock / 64;
     __shared__ uint32_t lds[Stack::kShort * Stack::kWords * kBlock];
     __shared__ float red[kAdjoint ? kMaxParams * 4 * kWaves : 1];
@@ -141,50 +153,80 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8, 8))) 
                 sum = sum + diff_path_radiance<kSpheres>(a, o, d, rng0, st);
                 continue;
             }
-            // Trace once, keeping each vertex on this thread's tape.
-            V3 L = v3(0.0f, 0.0f, 0.0f);
-            {
-                V3 Tf = v3(1.0f, 1.0f, 1.0f);
+            if (kMode == 1) {
+                // Replay: one trace, parameter derivatives carried forward.
+                float D[kMaxParams][3];
+#pragma unroll
+                for (int q = 0; q < kMaxParams; ++q) D[q][0] = D[q][1] = D[q][2] = 0.0f;
+                V3 T = v3(1.0f, 1.0f, 1.0f);
                 uint32_t rng = rng0;
                 for (uint32_t b = 0; b < a.max_depth; ++b) {
                     Vertex vx = diff_vertex<kSpheres>(a, o, d, rng, st);
-                    L = L + Tf * vx.emit;
-                    float4 *t = a.tape + 2u * ((size_t)b * a.tape_stride + tid);
-                    t[0] = make_float4(vx.emit.x, vx.emit.y, vx.emit.z, (float)vx.param);  // -1..7, exact
-                    t[1] = make_float4(vx.att.x, vx.att.y, vx.att.z, vx.cont ? 1.0f : 0.0f);
+#pragma unroll
+                    for (int q = 0; q < kMaxParams; ++q) {
+                        g[q][0] += vx.emit.x * D[q][0];
+                        g[q][1] += vx.emit.y * D[q][1];
+                        g[q][2] += vx.emit.z * D[q][2];
+                    }
                     if (!vx.cont) break;
-                    Tf = Tf * vx.att;
+                    const V3 dT = delta * T;
+#pragma unroll
+                    for (int q = 0; q < kMaxParams; ++q) {
+                        D[q][0] *= vx.att.x;
+                        D[q][1] *= vx.att.y;
+                        D[q][2] *= vx.att.z;
+                        if (q == vx.param) {
+                            D[q][0] += dT.x;
+                            D[q][1] += dT.y;
+                            D[q][2] += dT.z;
+                            g[q][3] += (dT.x + dT.y) + dT.z;
+                        }
+                    }
+                    T = T * vx.att;
+                    o = vx.p;
+                    d = vx.dir;
+                }
+                continue;
+            }
+            // Tape: trace once, keeping each vertex on this thread's tape.
+            uint32_t n = 0;
+            {
+                V3 T = v3(1.0f, 1.0f, 1.0f);
+                uint32_t rng = rng0;
+                for (uint32_t b = 0; b < a.max_depth; ++b) {
+                    Vertex vx = diff_vertex<kSpheres>(a, o, d, rng, st);
+                    float4 *t = a.tape + 3u * ((size_t)b * a.tape_stride + tid);
+                    t[0] = make_float4(vx.emit.x, vx.emit.y, vx.emit.z, (float)vx.param);  // -1..7, exact
+                    t[1] = make_float4(vx.att.x, vx.att.y, vx.att.z, 0.0f);
+                    t[2] = make_float4(T.x, T.y, T.z, 0.0f);
+                    n = b + 1u;
+                    if (!vx.cont) break;
+                    T = T * vx.att;
                     o = vx.p;
                     d = vx.dir;
                 }
             }
-            // Replay the path from the tape: the values a second trace of the
-            // same PCG stream would recompute, bit for bit.
-            V3 T = v3(1.0f, 1.0f, 1.0f), R = L;
-            for (uint32_t b = 0; b < a.max_depth; ++b) {
-                const float4 *t = a.tape + 2u * ((size_t)b * a.tape_stride + tid);
-                const float4 t0 = t[0], t1 = t[1];
-                Vertex vx;
-                vx.emit = xyz(t0);
-                vx.param = (int)t0.w;
-                vx.att = xyz(t1);
-                vx.cont = t1.w != 0.0f;
-                if (!vx.cont) break;
-                const V3 Rn = v3(safe_div(R.x - vx.emit.x, vx.att.x), safe_div(R.y - vx.emit.y, vx.att.y),
-                                 safe_div(R.z - vx.emit.z, vx.att.z));
-                if (vx.param >= 0) {
+            // Walk the tape backwards: R is the radiance arriving at vertex k
+            // from beyond it (0 past the last vertex; att is 0 where the path
+            // ends, so R restarts at that vertex's emission).
+            V3 R = v3(0.0f, 0.0f, 0.0f);
+            for (uint32_t b = n; b-- > 0u;) {
+                const float4 *t = a.tape + 3u * ((size_t)b * a.tape_stride + tid);
+                const float4 t0 = t[0], t1 = t[1], t2 = t[2];
+                const int param = (int)t0.w;
+                if (param >= 0) {
+                    const V3 dT = delta * xyz(t2);
 #pragma unroll
                     for (int q = 0; q < kMaxParams; ++q) {
-                        if (q == vx.param) {
-                            g[q][0] += delta.x * (T.x * Rn.x);
-                            g[q][1] += delta.y * (T.y * Rn.y);
-                            g[q][2] += delta.z * (T.z * Rn.z);
-                            g[q][3] += dot(delta, T);
+                        if (q == param) {
+                            g[q][0] += dT.x * R.x;
+                            g[q][1] += dT.y * R.y;
+                            g[q][2] += dT.z * R.z;
+                            g[q][3] += (dT.x + dT.y) + dT.z;
                         }
                     }
                 }
-                T = T * vx.att;
-                R = Rn;
+                R = xyz(t0) + xyz(t1) * R;
             }
         }
         if (!kAdjoint) {
@@ -269,19 +311,26 @@ hipError_t launch_diff(const DiffArgs &a, bool adjoint, hipStream_t s) {
     const uint32_t grid = diff_grid(wh, a.groups);
     const bool sph = a.scene.sroot != kNone;
     if (adjoint) {
-        if (sph)
-            hipLaunchKernelGGL((k_diff<kDiffBlock, true, true>), dim3(grid), dim3(kDiffBlock), 0, s, a);
-        else
-            hipLaunchKernelGGL((k_diff<kDiffBlock, false, true>), dim3(grid), dim3(kDiffBlock), 0, s, a);
+        if (a.tape) {
+            if (sph)
+                hipLaunchKernelGGL((k_diff<kDiffBlock, true, 2>), dim3(grid), dim3(kDiffBlock), 0, s, a);
+            else
+                hipLaunchKernelGGL((k_diff<kDiffBlock, false, 2>), dim3(grid), dim3(kDiffBlock), 0, s, a);
+        } else {
+            if (sph)
+                hipLaunchKernelGGL((k_diff<kDiffBlock, true, 1>), dim3(grid), dim3(kDiffBlock), 0, s, a);
+            else
+                hipLaunchKernelGGL((k_diff<kDiffBlock, false, 1>), dim3(grid), dim3(kDiffBlock), 0, s, a);
+        }
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
         if (a.n_params > 0u)
             hipLaunchKernelGGL(k_diff_sum, dim3(a.n_params * 4u), dim3(256), 0, s, a.partial, grid, a.grad);
     } else {
         if (sph)
-            hipLaunchKernelGGL((k_diff<kDiffBlock, true, false>), dim3(grid), dim3(kDiffBlock), 0, s, a);
+            hipLaunchKernelGGL((k_diff<kDiffBlock, true, 0>), dim3(grid), dim3(kDiffBlock), 0, s, a);
         else
-            hipLaunchKernelGGL((k_diff<kDiffBlock, false, false>), dim3(grid), dim3(kDiffBlock), 0, s, a);
+            hipLaunchKernelGGL((k_diff<kDiffBlock, false, 0>), dim3(grid), dim3(kDiffBlock), 0, s, a);
         if (a.groups > 1u) {
             hipError_t e = hipGetLastError();
             if (e != hipSuccess) return e;

@@ -55,6 +55,7 @@ struct mcrt_ctx {
     DevScene scene{};
     uint32_t n_tris = 0, n_spheres = 0;
     bool nee = false;  // next-event estimation (mcrt_set_nee)
+    int diff_mode = MCRT_DIFF_REPLAY;  // mcrt_diff_set_mode
     // work buffers
     DevBuf A[2], B[2], C[2], H, samples;
     DevBuf P[2], S_o, S_d, S_c;  // next-event estimation path pdf and shadow queue
@@ -1155,9 +1156,13 @@ extern "C" int mcrt_diff_adjoint_dev(mcrt_ctx *c, const CameraUniform *cam, uint
     if (!d_image_dev || !out_grad_dev) return fail(c, MCRT_ERR_BAD_ARG, "null pointer");
     const uint32_t grid = diff_grid(width * height, a.groups);
     if ((rc = ensure(c, c->diff_partial, (size_t)grid * kMaxParams * 4 * sizeof(float)))) return rc;
-    a.tape_stride = grid * 256u;
-    if ((rc = ensure(c, c->diff_tape, (size_t)a.tape_stride * max_depth * 2 * sizeof(float4)))) return rc;
-    a.tape = (float4 *)c->diff_tape.p;
+    if (c->diff_mode == MCRT_DIFF_TAPE) {
+        a.tape_stride = grid * 256u;
+        if ((rc = ensure(c, c->diff_tape, (size_t)a.tape_stride * max_depth * 3 * sizeof(float4)))) return rc;
+        a.tape = (float4 *)c->diff_tape.p;
+    } else {
+        a.tape = nullptr;  // replay: no tape
+    }
     a.d_image = (const float4 *)d_image_dev;
     a.partial = (float *)c->diff_partial.p;
     a.grad = out_grad_dev;
@@ -1165,6 +1170,15 @@ extern "C" int mcrt_diff_adjoint_dev(mcrt_ctx *c, const CameraUniform *cam, uint
     c->last_stream = s;
     Timed t(c, "diff_adjoint", s);
     HIP_TRY(c, launch_diff(a, true, s));
+    return MCRT_OK;
+}
+
+extern "C" int mcrt_diff_set_mode(mcrt_ctx *c, int mode) {
+    if (!c) return MCRT_ERR_BAD_ARG;
+    if (mode != MCRT_DIFF_REPLAY && mode != MCRT_DIFF_TAPE)
+        return fail(c, MCRT_ERR_BAD_ARG, "unknown adjoint mode %d", mode);
+    c->diff_mode = mode;
+    if (mode == MCRT_DIFF_REPLAY) release(c->diff_tape);
     return MCRT_OK;
 }
 

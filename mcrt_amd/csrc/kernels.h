@@ -138,7 +138,8 @@ struct DiffArgs {
     float4 *image_part;      // primal with groups > 1: groups x W*H radiance sums
     const float4 *d_image;   // adjoint: dLoss/dimage
     float *partial;          // adjoint: per-block sums, kMaxParams*4 per block
-    float4 *tape;            // adjoint: per-thread path vertices, 2 float4 x max_depth x tape_stride
+    float4 *tape;            // tape adjoint: per-thread vertices, 3 float4 x max_depth x tape_stride;
+                             // null selects the replay adjoint
     uint32_t tape_stride;    // adjoint: threads in the grid
     float *grad;             // adjoint: n_params*4
     uint32_t *ovf;

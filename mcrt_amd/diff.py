@@ -3,8 +3,8 @@
 * :class:`DiffRender` — a thin ``torch.autograd.Function``: forward calls
   ``mcrt_diff_render_dev``, backward calls ``mcrt_diff_adjoint_dev`` with the
   incoming image gradient.  Both run on the tensors' device memory and the
-  current stream; torch records nothing but the Function's inputs (the
-  adjoint keeps its own per-thread vertex tape inside the library).
+  current stream; torch records nothing but the Function's inputs, and the
+  adjoint re-traces its paths (path replay) instead of reading a tape.
 * :func:`inverse_workload` — the displaced-icosphere scene whose 8 icosphere
   materials become parameter materials (RGB albedo + scalar emission).
 * :func:`optimize` — Adam on the L2 image loss against a target rendered with
@@ -152,19 +152,28 @@ def _main(argv=None) -> int:
     ap = argparse.ArgumentParser(prog="python -m mcrt_amd.diff")
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--log-every", type=int, default=0)
+    ap.add_argument("--mode", choices=["replay", "tape"], default="replay",
+                    help="adjoint form: replay (re-trace, no tape) or tape")
     a = ap.parse_args(argv)
     torch.cuda.init()
     inv = inverse_workload()
     with Context(0) as ctx:
         ctx.upload_scene(inv.work.buffers)
+        ctx.set_diff_mode(a.mode)
         target_spp = 8 * inv.work.spp
         optimize(ctx, inv, steps=2, target_spp=target_spp)  # warm-up
         torch.cuda.synchronize()
         t0 = time.perf_counter()
+        ctx.set_profiling(True)
+        ctx.reset_stats()
         _, hist = optimize(ctx, inv, steps=a.steps, target_spp=target_spp, log_every=a.log_every)
         torch.cuda.synchronize()
         el = time.perf_counter() - t0
-    print(json.dumps({"steps": a.steps, "total_s": round(el, 3), "ms_per_step": round(el / a.steps * 1e3, 3),
+        fwd_ms, nf = ctx.kernel_stats("diff_primal")
+        adj_ms, na = ctx.kernel_stats("diff_adjoint")
+        ctx.set_profiling(False)
+    print(json.dumps({"mode": a.mode, "steps": a.steps,
+                      "primal_ms": round(fwd_ms / max(nf, 1), 3), "adjoint_ms": round(adj_ms / max(na, 1), 3), "total_s": round(el, 3), "ms_per_step": round(el / a.steps * 1e3, 3),
                       "albedo_err_first": {"max": round(hist[0][1], 4), "mean": round(hist[0][2], 4)},
                       "albedo_err_last": {"max": round(hist[-1][1], 4), "mean": round(hist[-1][2], 4)}}),
           flush=True)

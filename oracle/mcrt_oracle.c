@@ -979,10 +979,12 @@ static void render_run(const OracleScene *s, const Emitters *E, const CameraUnif
 /* ------------------------------------------------------------------------ */
 /* Differentiable-render reference (extended estimator, see the header).      */
 /* Path geometry in FP32 exactly as oracle_render; radiance and gradients in  */
-/* FP64.  The gradient is forward mode by counting: a radiance term           */
-/* t = T_k * Le_k contains a_m once per earlier vertex on material m, so      */
-/* dt/da_m[c] = t[c] * n_m / a_m[c]; dL/dE_p sums T_k over vertices on p.     */
-/* This is an independent derivation from the GPU's reverse replay.           */
+/* FP64.  The gradient is the power rule per radiance term: t = T_k * Le_k    */
+/* with T_k = a_m^{n_m} * O_m, where n_m counts the earlier vertices on       */
+/* parameter material m and O_m multiplies every other earlier attenuation,   */
+/* so dt/da_m[c] = n_m * a_m[c]^{n_m - 1} * O_m[c] * Le[c] (no division, so   */
+/* a zero albedo is exact); dL/dE_p sums T_k over vertices on p.  This is an  */
+/* independent derivation from the GPU kernels' product-rule accumulation.    */
 /* ------------------------------------------------------------------------ */
 #define ORACLE_MAX_PARAMS 8
 
@@ -1000,14 +1002,23 @@ typedef struct {
 } DiffJob;
 
 static void diff_term(DiffJob *j, const double T[3], const double Le[3], const int counts[],
-                      const double delta[3], double L[3]) {
+                      double other[][3], const double delta[3], double L[3]) {
     double t[3] = {T[0] * Le[0], T[1] * Le[1], T[2] * Le[2]};
     L[0] += t[0]; L[1] += t[1]; L[2] += t[2];
     if (!j->d_img) return;
     for (uint32_t m = 0; m < j->n_params; ++m) {
         if (!counts[m]) continue;
         for (int c = 0; c < 3; ++c)
-            j->grad[4 * m + c] += delta[c] * t[c] * (double)counts[m] / j->params[4 * m + c];
+            j->grad[4 * m + c] += delta[c] * Le[c] * (double)counts[m] *
+                                  pow(j->params[4 * m + c], (double)(counts[m] - 1)) * other[m][c];
+    }
+}
+
+/* O_m *= att for every parameter m except `skip` (-1: all). */
+static void diff_other(double other[][3], uint32_t n_params, int skip, const double att[3]) {
+    for (uint32_t m = 0; m < n_params; ++m) {
+        if ((int)m == skip) continue;
+        other[m][0] *= att[0]; other[m][1] *= att[1]; other[m][2] *= att[2];
     }
 }
 
@@ -1039,12 +1050,14 @@ static void *diff_worker(void *arg) {
                 v3 d = normalize(sub(focal, o));
                 double T[3] = {1, 1, 1}, L[3] = {0, 0, 0};
                 int counts[ORACLE_MAX_PARAMS] = {0};
+                double other[ORACLE_MAX_PARAMS][3];
+                for (int m = 0; m < ORACLE_MAX_PARAMS; ++m) other[m][0] = other[m][1] = other[m][2] = 1.0;
                 for (uint32_t b = 0; b < j->depth; ++b) {
                     Hit h = world_hit_ref(s, o, d, MCRT_T_MIN, MCRT_T_MAX, NULL);
                     if (h.t < 0.0f) {
                         v3 e = sample_env(s, d);
                         double Le[3] = {e.x, e.y, e.z};
-                        diff_term(j, T, Le, counts, delta, L);
+                        diff_term(j, T, Le, counts, other, delta, L);
                         break;
                     }
                     uint32_t mi = h.mat < MAX_MAT_IDX ? h.mat : MAX_MAT_IDX;
@@ -1054,22 +1067,24 @@ static void *diff_worker(void *arg) {
                     if (p >= 0) {
                         const double *q = &j->params[4 * p];
                         double Le[3] = {q[3], q[3], q[3]};
-                        diff_term(j, T, Le, counts, delta, L);
+                        diff_term(j, T, Le, counts, other, delta, L);
                         if (j->d_img) j->grad[4 * p + 3] += delta[0] * T[0] + delta[1] * T[1] + delta[2] * T[2];
                         Scatter sr = scatter(&lam, d, &h, &rng);
                         att[0] = q[0]; att[1] = q[1]; att[2] = q[2];
+                        diff_other(other, j->n_params, p, att);
                         counts[p] += 1;
                         d = sr.dir;
                     } else if (m->type_pad[0] == MCRT_MAT_EMISSIVE) {
                         double Le[3] = {(double)(m->ior_pad[0] * m->albedo_fuzz[0]),
                                         (double)(m->ior_pad[0] * m->albedo_fuzz[1]),
                                         (double)(m->ior_pad[0] * m->albedo_fuzz[2])};
-                        diff_term(j, T, Le, counts, delta, L);
+                        diff_term(j, T, Le, counts, other, delta, L);
                         break;
                     } else {
                         Scatter sr = scatter(m, d, &h, &rng);
                         if (sr.absorbed) break;
                         att[0] = sr.att.x; att[1] = sr.att.y; att[2] = sr.att.z;
+                        diff_other(other, j->n_params, -1, att);
                         d = sr.dir;
                     }
                     T[0] *= att[0]; T[1] *= att[1]; T[2] *= att[2];

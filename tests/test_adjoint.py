@@ -59,6 +59,30 @@ def test_oracle_gradient_matches_fp64_central_differences(small):
     np.testing.assert_allclose(g, fd, rtol=1e-4, atol=1e-6 * scale)
 
 
+def test_oracle_gradient_at_zero_and_small_albedo(small):
+    # A channel at exactly 0 ends every path through it in that channel, yet
+    # its derivative T_k * R_{k+1} is not 0; a gradient formed by dividing by
+    # the albedo reports 0 or NaN there.  Central differences are valid across
+    # 0 (the estimator is a polynomial in each albedo).
+    inv, sc = small
+    w = inv.work
+    target = oracle.diff_render64(sc, w.camera, w.width, w.height, w.spp, 100, w.depth, inv.hidden, inv.pmap)
+    params = inv.init.astype(np.float64)
+    params[:, 3] = 0.3
+    params[0, 1] = 0.0
+    params[1, :3] = 0.0
+    params[2, 0] = 0.01
+    loss, img = _loss64(sc, inv, params, target)
+    g = oracle.diff_grad64(sc, w.camera, w.width, w.height, w.spp, 0, w.depth, params, inv.pmap,
+                           _d_image(img, target))
+    assert np.all(np.isfinite(g))
+    idx = [(0, 1), (0, 0), (1, 0), (1, 1), (1, 2), (1, 3), (2, 0), (2, 2), (3, 1)]
+    fd = _fd(sc, inv, params, target, idx)
+    for (i, c) in idx:
+        assert abs(fd[i, c]) > 0, (i, c)
+        assert abs(g[i, c] - fd[i, c]) <= 1e-4 * abs(fd[i, c]) + 1e-9, (i, c, g[i, c], fd[i, c])
+
+
 def test_zero_emission_matches_reference_estimator(small):
     inv, sc = small
     w = inv.work
@@ -124,6 +148,39 @@ def test_gpu_path_replay_gradient_matches_oracle_and_fd(ctx, mid):
     # Deterministic reduction: identical on a second call.
     g2 = ctx.diff_adjoint(w.camera, w.width, w.height, w.spp, 0, w.depth, params, inv.pmap, d_img)
     assert np.array_equal(g_gpu, g2)
+
+
+@pytest.mark.gpu
+def test_gpu_replay_and_tape_agree_and_handle_zero_albedo(ctx, mid):
+    """Both adjoint forms against the oracle's power-rule gradient, with one
+    albedo channel at 0 and one at 0.01 (a gradient that divides by the
+    albedo reports 0 at the first); replay vs tape within FP32 rounding."""
+    inv, sc = mid
+    w = inv.work
+    ctx.upload_scene(w.buffers)
+    target = oracle.diff_render64(sc, w.camera, w.width, w.height, w.spp, 100, w.depth, inv.hidden, inv.pmap)
+    params = inv.init.copy()
+    params[:, 3] = 0.3
+    params[0, 1] = 0.0
+    params[1, :3] = 0.0
+    params[2, 0] = 0.01
+    loss, img64 = _loss64(sc, inv, params.astype(np.float64), target)
+    d_img = np.zeros((w.height, w.width, 4), np.float32)
+    d_img[..., :3] = _d_image(img64, target)
+    g_cpu = oracle.diff_grad64(sc, w.camera, w.width, w.height, w.spp, 0, w.depth, params, inv.pmap,
+                               d_img[..., :3].astype(np.float64))
+    out = {}
+    for mode in ("tape", "replay"):
+        ctx.set_diff_mode(mode)
+        out[mode] = ctx.diff_adjoint(w.camera, w.width, w.height, w.spp, 0, w.depth, params, inv.pmap, d_img)
+    ctx.set_diff_mode("replay")
+    for mode, g in out.items():
+        rel = np.linalg.norm(g - g_cpu) / np.linalg.norm(g_cpu)
+        print(f"{mode}: |g - g_cpu| / |g_cpu| = {rel:.2e}")
+        assert rel < 1e-4
+        for (i, c) in ((0, 1), (1, 0), (1, 1), (1, 2), (2, 0)):
+            assert abs(g_cpu[i, c]) > 0 and abs(g[i, c] - g_cpu[i, c]) <= 1e-3 * abs(g_cpu[i, c]), (mode, i, c)
+    np.testing.assert_allclose(out["tape"], out["replay"], rtol=1e-5, atol=1e-7 * np.abs(g_cpu).max())
 
 
 @pytest.mark.gpu
