@@ -254,6 +254,22 @@ int mcrt_trace_closest_dev(mcrt_ctx *ctx, const float *origins_dev,
                            float tmax, uint32_t *out_id_dev, float *out_t_dev,
                            void *stream);
 
+/* mcrt_trace_closest_dev that allocates nothing.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, serving loops, harnesses that forbid hipMalloc).  The
+ * scene is still uploaded by mcrt_upload_scene; the call's own scratch (the
+ * persistent grid's work counter and its traversal-stack overflow) lives in a
+ * static 32 MiB device array that each GPU the library runs on holds until
+ * the process ends.  Deep trees shrink the persistent grid to fit it; a tree
+ * too deep for one block returns MCRT_ERR_LIMIT.  Every call on a GPU shares
+ * that GPU's array, so calls that overlap on one GPU (other streams, other
+ * contexts, other host threads) race: serialise them.  Results are those of
+ * mcrt_trace_closest_dev. */
+int mcrt_trace_closest_noalloc_dev(mcrt_ctx *ctx, const float *origins_dev,
+                                   const float *dirs_dev, uint32_t n, float tmin,
+                                   float tmax, uint32_t *out_id_dev, float *out_t_dev,
+                                   void *stream);
+
 /* Binned-SAH BVH builders, src/mesh.rs:82-275 and src/bvh.rs:239-463 (8 bins,
  * leaf <= 4, pre-order, left = i + 1).  Host-only; no device needed.
  * Outputs are caller-allocated: out_nodes with room for 2*n - 1 nodes,

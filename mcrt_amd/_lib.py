@@ -74,6 +74,7 @@ def load() -> C.CDLL:
         "mcrt_denoise_bilateral_dev": (C.c_int, [_vp, _vp, _u32, _u32, _vp, _vp, _vp, _vp]),
         "mcrt_trace_closest": (C.c_int, [_vp, _vp, _vp, _u32, _f32, _f32, _vp, _vp]),
         "mcrt_trace_closest_dev": (C.c_int, [_vp, _vp, _vp, _u32, _f32, _f32, _vp, _vp, _vp]),
+        "mcrt_trace_closest_noalloc_dev": (C.c_int, [_vp, _vp, _vp, _u32, _f32, _f32, _vp, _vp, _vp]),
         "mcrt_build_mesh_bvh": (C.c_int, [_vp, _u32, _vp, _u32, _vp, C.POINTER(_u32), _vp, _vp]),
         "mcrt_build_sphere_bvh": (C.c_int, [_vp, _u32, _vp, C.POINTER(_u32), _vp, _vp]),
         "mcrt_validate_bvh": (C.c_int, [_vp, _u32, _u32]),
@@ -116,7 +117,7 @@ def exported_symbols() -> list:
     return [
         "mcrt_create", "mcrt_destroy", "mcrt_last_error", "mcrt_upload_scene", "mcrt_render",
         "mcrt_render_dev", "mcrt_denoise_bilateral", "mcrt_denoise_bilateral_dev",
-        "mcrt_trace_closest", "mcrt_trace_closest_dev", "mcrt_build_mesh_bvh",
+        "mcrt_trace_closest", "mcrt_trace_closest_dev", "mcrt_trace_closest_noalloc_dev", "mcrt_build_mesh_bvh",
         "mcrt_build_sphere_bvh", "mcrt_validate_bvh", "mcrt_set_profiling", "mcrt_set_nee", "mcrt_kernel_stats", "mcrt_reset_stats",
         "mcrt_render_counters", "mcrt_mesh_voxel_chunks", "mcrt_diff_render", "mcrt_diff_render_dev",
         "mcrt_diff_adjoint", "mcrt_diff_adjoint_dev", "mcrt_diff_set_mode", "mcrt_render_aux_dev", "mcrt_denoiser_create",
@@ -302,6 +303,11 @@ class Context:
     def trace_closest_dev(self, o_ptr, d_ptr, n, tmin, tmax, id_ptr, t_ptr, stream_ptr) -> None:
         self._check(self._lib.mcrt_trace_closest_dev(self._h, o_ptr, d_ptr, n, tmin, tmax, id_ptr,
                                                      t_ptr, stream_ptr))
+
+    def trace_closest_noalloc_dev(self, o_ptr, d_ptr, n, tmin, tmax, id_ptr, t_ptr, stream_ptr) -> None:
+        """mcrt_trace_closest_dev without device allocation (static scratch)."""
+        self._check(self._lib.mcrt_trace_closest_noalloc_dev(self._h, o_ptr, d_ptr, n, tmin, tmax, id_ptr,
+                                                             t_ptr, stream_ptr))
 
     # --- differentiable render (extended estimator, see include/mcrt_abi.h) ---
     @staticmethod

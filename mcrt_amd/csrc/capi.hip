@@ -14,6 +14,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <random>
 #include <string>
 #include <vector>
@@ -26,6 +27,31 @@ using namespace mcrt;
 namespace {
 
 thread_local std::string g_create_error;
+
+// Scratch of mcrt_trace_closest_noalloc_dev: the first 256 B hold the
+// persistent grid's work counter, the rest the traversal-stack overflow
+// columns.  One copy per device, for the life of the process; host code uses
+// the device address from hipGetSymbolAddress (g_scratch), never this name.
+constexpr size_t kNoAllocScratchBytes = (size_t)32 << 20;
+constexpr size_t kNoAllocCounterBytes = 256;
+static __device__ __attribute__((used)) __align__(256) uint32_t scratch[kNoAllocScratchBytes / sizeof(uint32_t)];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch[kMaxDevices] = {};          // device address of `scratch`, per device
+hipError_t g_scratch_err[kMaxDevices] = {};  // result of its lookup
+
+// Looks up `scratch` on `device` (which must be current) once per process.
+hipError_t scratch_setup(int device) {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+    if (!g_scratch[device]) {
+        void *p = nullptr;
+        g_scratch_err[device] = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (g_scratch_err[device] == hipSuccess) g_scratch[device] = p;
+    }
+    return g_scratch[device] ? hipSuccess : g_scratch_err[device];
+}
 
 struct DevBuf {
     void *p = nullptr;
@@ -399,6 +425,9 @@ extern "C" int mcrt_create(mcrt_ctx **out_ctx, int device) {
         return MCRT_ERR_NO_DEVICE;
     }
     c->num_cus = prop.multiProcessorCount;
+    // The no-allocation trace's scratch: its device address, per device.  A
+    // failed lookup is reported by that entry point, not here.
+    (void)scratch_setup(device);
     *out_ctx = c;
     return MCRT_OK;
 }
@@ -991,6 +1020,60 @@ extern "C" int mcrt_trace_closest_dev(mcrt_ctx *c, const float *origins_dev, con
         Timed t(c, "trace_closest", s);
         HIP_TRY(c, launch_trace(a, grid, s));
     }
+    return MCRT_OK;
+}
+
+extern "C" int mcrt_trace_closest_noalloc_dev(mcrt_ctx *c, const float *origins_dev,
+                                              const float *dirs_dev, uint32_t n, float tmin, float tmax,
+                                              uint32_t *out_id_dev, float *out_t_dev, void *stream) {
+    int rc = bind(c);
+    if (rc) return rc;
+    if (!c->has_scene) return fail(c, MCRT_ERR_NO_SCENE, "trace called before mcrt_upload_scene");
+    if (n == 0) return MCRT_OK;
+    if (!origins_dev || !dirs_dev || !out_id_dev || !out_t_dev)
+        return fail(c, MCRT_ERR_BAD_ARG, "null ray or output pointer");
+    if (n > 0xFFFFFFFFu - 64u) return fail(c, MCRT_ERR_LIMIT, "too many rays");
+    hipError_t se = scratch_setup(c->device);
+    if (se != hipSuccess)
+        return fail(c, MCRT_ERR_HIP, "no-allocation scratch unavailable on device %d: %s", c->device,
+                    hipGetErrorString(se));
+    uint8_t *base = (uint8_t *)g_scratch[c->device];
+    // The persistent grid of mcrt_trace_closest_dev, shrunk until every
+    // thread's overflow column fits the scratch.
+    uint32_t grid = 0;
+    grid_for(c, 2, &grid);
+    const uint32_t block = (uint32_t)trace_block();
+    const uint32_t need = (n + 63u) / 64u;
+    grid = std::max<uint32_t>(1, std::min(grid, (need + block / 64u - 1u) / (block / 64u)));
+    uint32_t sh = 0, words = 0;
+    stack_layout(c->scene.stack_cull != 0, &sh, &words);
+    const uint32_t depth = c->scene.stack_depth;
+    const size_t per_thread = depth > sh ? (size_t)(depth - sh) * words * sizeof(uint32_t) : 0;
+    if (per_thread) {
+        const size_t fit = (kNoAllocScratchBytes - kNoAllocCounterBytes) / (per_thread * block);
+        if (fit == 0)
+            return fail(c, MCRT_ERR_LIMIT,
+                        "BVH depth %u needs %zu B of traversal-stack scratch per block of %u threads; the "
+                        "no-allocation trace holds %zu B",
+                        depth, per_thread * block, block, kNoAllocScratchBytes - kNoAllocCounterBytes);
+        grid = (uint32_t)std::min<size_t>(grid, fit);
+    }
+    hipStream_t s = (hipStream_t)stream;
+    c->last_stream = s;
+    TraceArgs a{};
+    a.scene = c->scene;
+    a.origins = origins_dev;
+    a.dirs = dirs_dev;
+    a.n = n;
+    a.tmin = tmin;
+    a.tmax = tmax;
+    a.out_id = out_id_dev;
+    a.out_t = out_t_dev;
+    a.work = (uint32_t *)base;
+    a.ovf = per_thread ? (uint32_t *)(base + kNoAllocCounterBytes) : nullptr;
+    a.ovf_stride = grid * block;
+    HIP_TRY(c, launch_fill_u32(a.work, 1u, 0u, s));  // a fill kernel, not a memset
+    HIP_TRY(c, launch_trace(a, grid, s));
     return MCRT_OK;
 }
 
