@@ -2,44 +2,65 @@
 """Benchmark of the render hot path on one MI355X.
 
 One step = one full progressive render of the workload (every frame of its
-spp, every bounce) through ``mcrt_render_dev`` on device-resident buffers.
-The scene is uploaded before timing; the timed region holds only GPU work.
+spp, every bounce) through ``mcrt_render_dev`` on device-resident buffers,
+plus the bilateral pass.  The scene is uploaded before timing; the timed
+region holds only GPU work.
 
-Output: one JSON line with Msamples/s, the roofline of the dominant kernel
-(``extend``, the closest-hit pass) measured with HIP events on its launch
-stream, and the CPU oracle timed on a bounded sample of the same workload.
+Output: one JSON line.  ``value`` is Msamples/s of the 1920 x 1080 voxel
+terrain at 16 spp, depth 6, with the bilateral denoiser (the 1080p
+configuration of the metric).  Beside it:
 
-    python bench.py [--steps K] [--warmup W] [--workload c3|c2|c1]
+- ``roofline``: the dominant kernel (``extend``, closest hit).  Its launch
+  time comes from HIP events on its stream; its HBM bytes per launch
+  (FETCH_SIZE, WRITE_SIZE) and its VALU issue and lane counts (SQ counters)
+  from ``rocprofv3 --pmc`` passes that this script runs on itself (``--probe``)
+  in the same invocation, each counter scaled by what it reads for a copy
+  of known size in the same pass;
+- ``peaks_measured``: HBM copy bandwidth and dense BF16 MFMA rate measured
+  on the device (``mcrt_measure_peaks``);
+- ``cornell_c1`` (256 x 256, 16 spp, depth 8) and ``icosphere_c2`` (1024 x
+  1024, 64 spp, depth 8) render lines, relMSE against the 65,536-spp FP64
+  oracle where its fixture exists;
+- ``microbench``: 16M closest-hit rays on the icosphere scene;
+- ``denoise_eval``: time to relMSE <= 1e-2, learned vs bilateral (1080p);
+- ``inverse``: 200 Adam steps through the path-replay adjoint;
+- ``cpu_baseline``: the CPU oracle on the host's cores.
 
-Default workload: the voxel terrain at 1920 x 1080, 16 spp, depth 6, with the
-bilateral denoiser in every step (the 1080p configuration of the metric).
-The 16M-ray closest-hit microbenchmark runs on the displaced-icosphere scene.
-
-Beside the main figure the line carries:
-- ``denoise_eval``: wall time of render + denoise until relMSE <= 1e-2 on the
-  1920 x 1080 icosphere frame, learned (BF16 MFMA) vs bilateral vs none, and
-  the learned denoiser's inference rate against the BF16 MFMA peak;
-- ``inverse``: 200 Adam steps of inverse rendering through the path-replay
-  adjoint (icosphere scene, 512 x 512, 32 spp, depth 6, 8 albedos).
+    python bench.py [--steps K] [--warmup W]
 """
 from __future__ import annotations
 
 import argparse
+import csv
+import glob
 import json
 import os
+import shutil
+import signal
+import statistics
+import subprocess
 import sys
+import tempfile
 import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "Msamples/s at 1080p; time to relMSE≤1e-2, learned vs bilateral denoise"
-HBM_PEAK_GBS = 8000.0
-MFMA_BF16_PEAK_TFLOPS = 2500.0
-# Measured bytes between L2 and memory per k_extend launch (FETCH_SIZE x 2 +
-# WRITE_SIZE x 1, factors from a 1 GiB copy in the same rocprofv3 --pmc runs),
-# per workload at its default size; profiles/traffic_extend_voxel_c3_1080p.md.
-EXTEND_TRAFFIC_BYTES = {"voxel_c3": 12.358e9}
+HBM_PEAK_GBS = 8000.0          # MI355X HBM3E, published
+MFMA_BF16_PEAK_TFLOPS = 2500.0  # dense BF16, published
+FP32_VALU_PEAK_TFLOPS = 78.6    # non-packed FP32 FMA: 256 CUs x 128 FLOP/clk x 2.4 GHz
+N_XCD = 8
+COPY_PROBE_BYTES = 1 << 30
+EXTEND_C3 = "k_extend<256, false, true>"   # culling stack (the terrain spills the L2)
+EXTEND_C2 = "k_extend<256, false, false>"  # plain stack (cache-resident scene)
+COPY_KERNEL = "k_copy_probe"
+PMC_PASSES = [
+    ["FETCH_SIZE"],
+    ["WRITE_SIZE", "TCC_HIT_sum", "TCC_MISS_sum"],
+    ["SQ_WAVES", "SQ_INSTS_VALU", "SQ_THREAD_CYCLES_VALU", "SQ_ACTIVE_INST_VALU", "SQ_INSTS_VMEM_RD",
+     "SQ_WAVE_CYCLES", "GRBM_GUI_ACTIVE", "TA_BUSY_avr"],
+]
 
 
 def parse():
@@ -50,15 +71,18 @@ def parse():
     p.add_argument("--workload", default="c3")
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--no-roofline", action="store_true")
+    p.add_argument("--no-counters", action="store_true", help="skip the rocprofv3 counter passes")
     p.add_argument("--no-microbench", action="store_true")
+    p.add_argument("--no-lines", action="store_true", help="skip the C1 and C2 render lines")
     p.add_argument("--cpu-seconds", type=float, default=12.0)
     p.add_argument("--no-denoise-eval", action="store_true",
                    help="skip the 1080p time-to-relMSE comparison of the learned and bilateral denoisers")
     p.add_argument("--no-inverse", action="store_true",
                    help="skip the 200-step inverse-rendering (path-replay adjoint + Adam) timing")
-    p.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
-                   help="first copy MIB MiB device-to-device three times (a plain copy of known size that "
-                        "calibrates the FETCH_SIZE / WRITE_SIZE counters)")
+    p.add_argument("--counters-dir", default=None,
+                   help="keep the rocprofv3 counter CSVs here (default: a temporary directory)")
+    p.add_argument("--probe", default=None, metavar="DIR",
+                   help="internal: run the counter probe on the scenes cached in DIR and exit")
     return p.parse_args()
 
 
@@ -67,13 +91,70 @@ def log(msg):
     sys.stderr.flush()
 
 
+# ---------------------------------------------------------------------------
+# Scene cache shared with the counter probe (a structured-array .npz, no pickle)
+# ---------------------------------------------------------------------------
+def save_workload(w, path):
+    import numpy as np
+
+    sb = w.buffers
+    np.savez(path, **{k: v for k, v in sb.__dict__.items() if isinstance(v, np.ndarray)},
+             n_real_spheres=sb.n_real_spheres, n_real_tris=sb.n_real_tris, camera=w.camera,
+             size=np.array([w.width, w.height, w.spp, w.depth]), name=np.array(w.name),
+             denoise=np.array(w.denoise or ""))
+
+
+def load_workload(path):
+    import numpy as np
+
+    from mcrt_amd import configs
+    from mcrt_amd.scene import SceneBuffers
+
+    z = np.load(path, allow_pickle=False)
+    fields = {k: z[k] for k in ("spheres", "sphere_orig", "sphere_bvh", "vertices", "triangles", "tri_orig",
+                                "mesh_bvh", "materials", "env", "emissive_spheres")}
+    sb = SceneBuffers(**fields, n_real_spheres=int(z["n_real_spheres"]), n_real_tris=int(z["n_real_tris"]))
+    W, H, spp, depth = (int(v) for v in z["size"])
+    return configs.Workload(str(z["name"]), None, sb, z["camera"], W, H, spp, depth, str(z["denoise"]) or None)
+
+
+def render_step(ctx, w, accum, gbuf, denoised, sp):
+    ctx.render_dev(w.camera, w.width, w.height, w.spp, 0, w.depth, None, accum.data_ptr(), gbuf.data_ptr(), sp)
+    if denoised is not None:
+        ctx.denoise_bilateral_dev(None, w.width, w.height, accum.data_ptr(), gbuf.data_ptr(), denoised.data_ptr(),
+                                  sp)
+
+
+def probe(cache_dir):
+    """Counter probe: one copy of known size, then one step of each cached
+    workload.  Run under rocprofv3 --pmc by counter_passes()."""
+    import torch
+
+    from mcrt_amd._lib import Context, measure_peaks
+
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    measure_peaks(0, COPY_PROBE_BYTES)
+    with Context(0) as ctx:
+        for name in ("c3", "c2"):
+            w = load_workload(os.path.join(cache_dir, f"{name}.npz"))
+            ctx.upload_scene(w.buffers)
+            accum = torch.empty((w.height, w.width, 4), dtype=torch.float32, device=dev)
+            gbuf = torch.empty_like(accum)
+            den = torch.empty_like(accum) if w.denoise == "bilateral" else None
+            render_step(ctx, w, accum, gbuf, den, torch.cuda.current_stream(dev).cuda_stream)
+            torch.cuda.synchronize()
+    return 0
+
+
 def main():
     a = parse()
+    if a.probe:
+        return probe(a.probe)
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); the multi-GPU design is "
               "described but not built", file=sys.stderr)
         return 2
-    import numpy as np
     import torch
 
     from mcrt_amd import configs
@@ -83,30 +164,16 @@ def main():
     w = configs.by_name(a.workload)
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
-    if a.copy_calibration > 0:
-        n = a.copy_calibration << 18  # float32 elements
-        src = torch.ones(n, dtype=torch.float32, device=dev)
-        dst = torch.empty_like(src)
-        for _ in range(3):
-            dst.copy_(src)
-        torch.cuda.synchronize()
-        log(f"copy calibration: 3 x {a.copy_calibration} MiB read + written")
-        del src, dst
     ctx = Context(0)
     ctx.upload_scene(w.buffers)
     accum = torch.empty((w.height, w.width, 4), dtype=torch.float32, device=dev)
     gbuf = torch.empty_like(accum)
     stream = torch.cuda.current_stream(dev)
     sp = stream.cuda_stream
-
     denoised = torch.empty_like(accum) if w.denoise == "bilateral" else None
 
     def step():
-        ctx.render_dev(w.camera, w.width, w.height, w.spp, 0, w.depth, None, accum.data_ptr(),
-                       gbuf.data_ptr(), sp)
-        if denoised is not None:
-            ctx.denoise_bilateral_dev(None, w.width, w.height, accum.data_ptr(), gbuf.data_ptr(),
-                                      denoised.data_ptr(), sp)
+        render_step(ctx, w, accum, gbuf, denoised, sp)
 
     log(f"warmup {a.warmup}")
     for _ in range(a.warmup):
@@ -147,16 +214,34 @@ def main():
         "mrays_per_s": round(segments / sec / 1e6, 3),
         "segments_per_step": int(segments),
     }
+    relmse_c3 = relmse_windows(w, accum, denoised, "c3_oracle_b.npz")
+    if relmse_c3:
+        out["relmse_vs_oracle_b"] = relmse_c3
 
+    lines = {}
+    if not a.no_lines:
+        lines["cornell_c1"] = c1_line(ctx)
+        lines["icosphere_c2"] = c2_line(ctx)
+        ctx.upload_scene(w.buffers)
+
+    counters = None
+    if not a.no_counters and not a.no_roofline:
+        counters = counter_passes(w, a.counters_dir)
+    peaks = measured_peaks()
+    out["peaks_measured"] = peaks
     if not a.no_roofline:
-        out["roofline"] = roofline(ctx, w, step, segments)
+        out["roofline"] = roofline(ctx, w, step, segments, counters, peaks)
+    if "icosphere_c2" in lines and counters and counters.get(EXTEND_C2):
+        lines["icosphere_c2"]["extend_counters"] = kernel_fractions(counters, EXTEND_C2,
+                                                                    lines["icosphere_c2"].get("extend_ms"))
+    out.update(lines)
     if not a.no_microbench:
         log("microbench: 16M rays on the icosphere scene")
         mb = configs.c2(width=64, height=64, spp=1)
         ctx.upload_scene(mb.buffers)
         out["microbench"] = microbench(ctx, mb)
     if not a.no_denoise_eval:
-        out["denoise_eval"] = denoise_eval(ctx)
+        out["denoise_eval"] = denoise_eval(ctx, peaks)
     if not a.no_inverse:
         out["inverse"] = inverse(ctx)
     if not a.no_cpu_baseline:
@@ -166,11 +251,140 @@ def main():
     return 0
 
 
-def roofline(ctx, w, step, segments):
-    """Dominant kernel: extend (closest hit).  Algorithmic bytes per segment
-    B_ray = 24 (ray in) + 8 (hit out) + 32 * nodes + 36 * triangles, with the
-    node and triangle counts of the reference-order traversal measured by the
-    CPU oracle on a 1-spp sample of the same render."""
+# ---------------------------------------------------------------------------
+# Counters (rocprofv3 --pmc on this script's --probe)
+# ---------------------------------------------------------------------------
+def _rocprofv3():
+    p = shutil.which("rocprofv3") or "/opt/rocm/bin/rocprofv3"
+    return p if os.path.exists(p) else None
+
+
+def counter_passes(w, keep_dir=None):
+    """Per-dispatch means of every counter, by kernel name (template
+    arguments included), over the three passes; None when rocprofv3 is
+    unavailable or a pass fails."""
+    import numpy as np
+
+    from mcrt_amd import configs
+
+    prof = _rocprofv3()
+    if not prof:
+        log("counters: rocprofv3 not found, skipped")
+        return None
+    work = tempfile.mkdtemp(prefix="mcrt_bench_")
+    out_dir = keep_dir or work
+    try:
+        save_workload(w, os.path.join(work, "c3.npz"))
+        save_workload(configs.c2(), os.path.join(work, "c2.npz"))
+        means = {}
+        for k, counters in enumerate(PMC_PASSES):
+            d = os.path.join(out_dir, f"pass{k}")
+            cmd = [prof, "--pmc", *counters, "--output-format", "csv", "-d", d, "-o", "p", "--",
+                   sys.executable, os.path.abspath(__file__), "--probe", work]
+            log(f"counters: pass {k + 1}/{len(PMC_PASSES)} ({' '.join(counters)})")
+            env = dict(os.environ, TMPDIR=os.environ.get("TMPDIR", "/tmp"))
+            p = subprocess.Popen(cmd, cwd="/tmp", env=env, stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
+                                 start_new_session=True)
+            try:
+                txt, _ = p.communicate(timeout=240)
+            except subprocess.TimeoutExpired:
+                os.killpg(p.pid, signal.SIGKILL)
+                p.communicate()
+                log(f"counters: pass {k + 1} timed out; counters skipped")
+                return None
+            files = glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True)
+            if p.returncode != 0 and not files:
+                log(f"counters: pass {k + 1} failed (rc {p.returncode}): {txt.decode(errors='replace')[-400:]}")
+                return None
+            for f in files:
+                acc = {}
+                for r in csv.DictReader(open(f)):
+                    acc.setdefault(r["Kernel_Name"], {}).setdefault(r["Counter_Name"], []).append(
+                        float(r["Counter_Value"]))
+                for kern, cs in acc.items():
+                    for c, vals in cs.items():
+                        means.setdefault(kern, {})[c] = (float(np.mean(vals)), len(vals))
+        by_short = {}
+        for kern, cs in means.items():
+            for short in (EXTEND_C3, EXTEND_C2, COPY_KERNEL, "k_shade"):
+                if short in kern:
+                    by_short.setdefault(short, {}).update(cs)
+        return by_short
+    finally:
+        if keep_dir is None:
+            shutil.rmtree(work, ignore_errors=True)
+        else:
+            for f in ("c3.npz", "c2.npz"):
+                try:
+                    os.remove(os.path.join(work, f))
+                except OSError:
+                    pass
+
+
+def traffic_factors(counters):
+    """Counter scale factors from the probe copy (1 GiB read + 1 GiB written
+    per dispatch): bytes per counted KiB."""
+    cp = counters.get(COPY_KERNEL, {})
+    if "FETCH_SIZE" not in cp or "WRITE_SIZE" not in cp:
+        return None
+    return (COPY_PROBE_BYTES / (cp["FETCH_SIZE"][0] * 1024.0), COPY_PROBE_BYTES / (cp["WRITE_SIZE"][0] * 1024.0))
+
+
+def kernel_fractions(counters, kern, avg_ms):
+    """HBM traffic and VALU figures of one kernel from the counter means."""
+    c = counters.get(kern, {})
+    f = traffic_factors(counters)
+    res = {}
+    if f and "FETCH_SIZE" in c and "WRITE_SIZE" in c:
+        traffic = c["FETCH_SIZE"][0] * 1024.0 * f[0] + c["WRITE_SIZE"][0] * 1024.0 * f[1]
+        res["traffic_bytes"] = round(traffic)
+        res["fetch_factor"], res["write_factor"] = round(f[0], 4), round(f[1], 4)
+        if avg_ms:
+            res["hbm_gbs"] = round(traffic / (avg_ms * 1e-3) / 1e9, 1)
+            res["hbm_frac"] = round(traffic / (avg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
+    if "TCC_HIT_sum" in c and "TCC_MISS_sum" in c:
+        h, m = c["TCC_HIT_sum"][0], c["TCC_MISS_sum"][0]
+        res["l2_hit_rate"] = round(h / max(h + m, 1.0), 4)
+    if "SQ_INSTS_VALU" in c and "GRBM_GUI_ACTIVE" in c:
+        import torch
+
+        simds = torch.cuda.get_device_properties(0).multi_processor_count * 4
+        cycles = c["GRBM_GUI_ACTIVE"][0] / N_XCD  # the counter sums the 8 XCDs
+        insts = c["SQ_INSTS_VALU"][0]
+        # one wave64 VALU instruction occupies a 16-lane SIMD for 4 cycles
+        res["valu_issue_frac"] = round(insts * 4.0 / (simds * cycles), 4)
+        res["valu_insts"] = round(insts)
+        if "SQ_THREAD_CYCLES_VALU" in c:
+            lanes = c["SQ_THREAD_CYCLES_VALU"][0] / max(insts, 1.0)
+            res["active_lanes_per_valu"] = round(lanes, 2)
+            res["lane_efficiency"] = round(lanes / 64.0, 4)
+            res["valu_lane_frac"] = round(res["valu_issue_frac"] * lanes / 64.0, 4)
+        if "TA_BUSY_avr" in c:
+            res["ta_busy_frac"] = round(c["TA_BUSY_avr"][0] / cycles, 4)
+        if "SQ_INSTS_VMEM_RD" in c:
+            res["vmem_rd_insts"] = round(c["SQ_INSTS_VMEM_RD"][0])
+        res["cycles_per_xcd"] = round(cycles)
+    return res
+
+
+def measured_peaks():
+    from mcrt_amd._lib import measure_peaks
+
+    log("peaks: copy and MFMA probes")
+    gbs, tflops = measure_peaks(0, COPY_PROBE_BYTES)
+    return {"hbm_copy_gbs": round(gbs, 1), "hbm_copy_frac_of_published": round(gbs / HBM_PEAK_GBS, 4),
+            "mfma_bf16_tflops": round(tflops, 1),
+            "mfma_bf16_frac_of_published": round(tflops / MFMA_BF16_PEAK_TFLOPS, 4),
+            "method": "mcrt_measure_peaks: float4 copy of 2 x 1 GiB, register-operand v_mfma_f32_16x16x32_bf16 "
+                      "loop on every SIMD; HIP events, best of several runs"}
+
+
+def roofline(ctx, w, step, segments, counters, peaks):
+    """Dominant kernel: extend (closest hit).  `achieved` is its HBM traffic
+    per launch (counters of the same invocation) over its mean launch time
+    (HIP events on its stream).  The reference-order byte model of the
+    traversal is reported beside it as a cache-reuse ratio, not as a
+    fraction: the 2.16M-triangle scene is served largely from L2 and MALL."""
     import torch
 
     import oracle
@@ -182,35 +396,150 @@ def roofline(ctx, w, step, segments):
     ms, launches = ctx.kernel_stats("extend")
     shade_ms, _ = ctx.kernel_stats("shade")
     ctx.set_profiling(False)
+    avg_ms = ms / max(launches, 1)
     log("counting reference traversal work with the oracle (1 spp sample)")
     sw, sh = w.width // 4, w.height // 4
     cam = __import__("mcrt_amd.camera", fromlist=["camera_for"]).camera_for(w.loaded, sw, sh)
     _, _, cnt = oracle.render(oracle.Scene(w.buffers), cam, sw, sh, 1, 0, w.depth, counts=True)
     nodes, tris, sph, seg = cnt
     b_ray = 24.0 + 8.0 + 32.0 * nodes / seg + 36.0 * tris / seg + 32.0 * sph / seg
-    total_bytes = b_ray * segments
-    achieved = total_bytes / (ms * 1e-3) / 1e9
-    avg_ms = ms / max(launches, 1)
-    traffic = EXTEND_TRAFFIC_BYTES.get(w.name)
-    return {
+    alg_gbs = b_ray * segments / (ms * 1e-3) / 1e9
+    res = {
         "kernel": "extend",
         "bound": "hbm",
-        "achieved": round(achieved, 1),
+        "achieved": None,
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
-        "frac": round(achieved / HBM_PEAK_GBS, 4),
-        "traffic": traffic,
-        "traffic_gbs": round(traffic / (avg_ms * 1e-3) / 1e9, 1) if traffic else None,
-        "traffic_frac": round(traffic / (avg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4) if traffic else None,
-        "algorithmic_bytes_per_launch": round(total_bytes / max(launches, 1)),
+        "frac": None,
+        "traffic": None,
         "avg_launch_ms": round(avg_ms, 4),
         "launches": int(launches),
-        "bytes_per_segment": round(b_ray, 1),
-        "nodes_per_segment": round(nodes / seg, 2),
-        "tris_per_segment": round(tris / seg, 2),
         "extend_ms_per_step": round(ms, 3),
         "shade_ms_per_step": round(shade_ms, 3),
+        "segments_per_launch": round(segments / max(launches, 1)),
+        "reference_order_model": {
+            "bytes_per_segment": round(b_ray, 1), "nodes_per_segment": round(nodes / seg, 2),
+            "tris_per_segment": round(tris / seg, 2), "algorithmic_gbs": round(alg_gbs, 1),
+            "algorithmic_over_hbm_peak": round(alg_gbs / HBM_PEAK_GBS, 3),
+            "note": "B_ray = 24 + 8 + 32 N_nodes + 36 N_tris with the oracle's reference-order counts; above 1 "
+                    "because the traversal set is reused from L2/MALL, so it is a reuse ratio, not a roofline "
+                    "fraction"},
     }
+    if counters and counters.get(EXTEND_C3 if w.name == "voxel_c3" else EXTEND_C2):
+        kf = kernel_fractions(counters, EXTEND_C3 if w.name == "voxel_c3" else EXTEND_C2, avg_ms)
+        if "traffic_bytes" in kf:
+            res["traffic"] = kf["traffic_bytes"]
+            res["achieved"] = kf["hbm_gbs"]
+            res["frac"] = kf["hbm_frac"]
+            res["frac_of_measured_copy"] = round(kf["hbm_gbs"] / peaks["hbm_copy_gbs"], 4)
+            res["bytes_per_segment"] = round(kf["traffic_bytes"] / max(segments / max(launches, 1), 1), 1)
+        res["counters"] = kf
+        res["counters_source"] = ("rocprofv3 --pmc passes of `bench.py --probe` in this invocation: "
+                                  + " | ".join(" ".join(p) for p in PMC_PASSES))
+    return res
+
+
+# ---------------------------------------------------------------------------
+# Render lines and relMSE against the FP64 oracle
+# ---------------------------------------------------------------------------
+def _rel_mse(x, y):
+    import numpy as np
+
+    x = np.asarray(x, np.float64)[..., :3]
+    y = np.asarray(y, np.float64)[..., :3]
+    return float(np.mean((x - y) ** 2 / (y * y + 1e-2)))
+
+
+def relmse_windows(w, accum, denoised, fixture):
+    import numpy as np
+
+    path = os.path.join(ROOT, "tests", "golden", fixture)
+    if not os.path.exists(path):
+        return None
+    z = np.load(path, allow_pickle=False)
+    acc = accum.cpu().numpy()
+    den = denoised.cpu().numpy() if denoised is not None else None
+    res = {"reference": f"{len(z['origins'])} windows {z['crops'].shape[1]}x{z['crops'].shape[2]}, "
+                        f"{int(z['spp'])} spp FP64 oracle"}
+    crops = z["crops"]
+    for key, img in (("render", acc), ("denoised", den)):
+        if img is None:
+            continue
+        parts = [img[y:y + crops.shape[1], x:x + crops.shape[2]] for (x, y) in z["origins"]]
+        res[key] = round(_rel_mse(np.stack(parts), crops), 6)
+    return res
+
+
+def _time_render(ctx, w, reps, warm=1):
+    import torch
+
+    dev = torch.device("cuda:0")
+    accum = torch.empty((w.height, w.width, 4), dtype=torch.float32, device=dev)
+    gbuf = torch.empty_like(accum)
+    sp = torch.cuda.current_stream(dev).cuda_stream
+    for _ in range(warm):
+        render_step(ctx, w, accum, gbuf, None, sp)
+    torch.cuda.synchronize()
+    times = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        render_step(ctx, w, accum, gbuf, None, sp)
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t0)
+    seg, _ = ctx.render_counters()
+    ctx.set_profiling(True)
+    ctx.reset_stats()
+    render_step(ctx, w, accum, gbuf, None, sp)
+    torch.cuda.synchronize()
+    ext_ms, ext_n = ctx.kernel_stats("extend")
+    ctx.set_profiling(False)
+    return statistics.median(times), seg, accum, gbuf, ext_ms / max(ext_n, 1), ext_n
+
+
+def c1_line(ctx):
+    """Cornell box 256 x 256, 16 spp, depth 8: Msamples/s (median of 20),
+    relMSE against the FP64 oracle's full frame, 8-bit PSNR against the FP32
+    oracle's render of the same frames."""
+    import numpy as np
+
+    import oracle
+    from mcrt_amd import configs, headless, regression
+
+    log("line: cornell_c1 256x256 16 spp")
+    w = configs.c1()
+    ctx.upload_scene(w.buffers)
+    t, seg, accum, gbuf, ext_ms, ext_n = _time_render(ctx, w, 20, warm=3)
+    res = {"width": w.width, "height": w.height, "spp": w.spp, "max_depth": w.depth,
+           "triangles": int(len(w.buffers.triangles)), "ms_per_render": round(t * 1e3, 3),
+           "msamples_per_s": round(w.width * w.height * w.spp / t / 1e6, 2),
+           "mrays_per_s": round(seg / t / 1e6, 2), "extend_ms": round(ext_ms, 4), "extend_launches": int(ext_n),
+           "bound": "launch: 36 triangles and 1M samples in about 1 ms, so kernel launches and persistent-grid "
+                    "drains, not memory or VALU, set the time"}
+    acc = accum.cpu().numpy()
+    ref, _ = oracle.render(oracle.Scene(w.buffers), w.camera, w.width, w.height, w.spp, 0, w.depth)
+    cmp = regression.compare_images(headless.to_rgba8(acc), headless.to_rgba8(ref))
+    res["vs_oracle_fp32_8bit"] = {"mse": round(cmp.mse, 8), "psnr_db": round(cmp.psnr, 2) if np.isfinite(cmp.psnr)
+                                  else "inf"}
+    path = os.path.join(ROOT, "tests", "golden", "c1_oracle_b.npz")
+    if os.path.exists(path):
+        z = np.load(path, allow_pickle=False)
+        res["relmse_vs_oracle_b"] = round(_rel_mse(acc, z["image"]), 6)
+        res["oracle_b"] = f"full frame, {int(z['spp'])} spp FP64"
+    return res
+
+
+def c2_line(ctx):
+    """Displaced icosphere in the Cornell box, 1024 x 1024, 64 spp, depth 8."""
+    from mcrt_amd import configs
+
+    log("line: icosphere_c2 1024x1024 64 spp")
+    w = configs.c2()
+    ctx.upload_scene(w.buffers)
+    t, seg, _, _, ext_ms, ext_n = _time_render(ctx, w, 3, warm=1)
+    return {"width": w.width, "height": w.height, "spp": w.spp, "max_depth": w.depth,
+            "triangles": int(len(w.buffers.triangles)), "ms_per_render": round(t * 1e3, 3),
+            "msamples_per_s": round(w.width * w.height * w.spp / t / 1e6, 2),
+            "mrays_per_s": round(seg / t / 1e6, 2), "extend_ms": round(ext_ms, 4), "extend_launches": int(ext_n)}
 
 
 def microbench(ctx, w, n=1 << 24, reps=5):
@@ -230,8 +559,7 @@ def microbench(ctx, w, n=1 << 24, reps=5):
     sp = torch.cuda.current_stream(dev).cuda_stream
 
     def go():
-        ctx.trace_closest_dev(to.data_ptr(), td.data_ptr(), n, 1e-4, 1e9, ids.data_ptr(),
-                              ts.data_ptr(), sp)
+        ctx.trace_closest_dev(to.data_ptr(), td.data_ptr(), n, 1e-4, 1e9, ids.data_ptr(), ts.data_ptr(), sp)
 
     go()
     torch.cuda.synchronize()
@@ -251,7 +579,7 @@ def microbench(ctx, w, n=1 << 24, reps=5):
             "hit_fraction": round(hit, 4)}
 
 
-def denoise_eval(ctx):
+def denoise_eval(ctx, peaks):
     """Wall time of render + denoise to relMSE <= 1e-2 on the 1920 x 1080
     icosphere frame: learned (committed weights) vs bilateral vs none; relMSE
     on fixed windows against the 65,536-spp FP64 reference of those windows."""
@@ -268,13 +596,15 @@ def denoise_eval(ctx):
     if dn_nee is not None:
         dn_nee.close()
     res["infer_mfma"] = {"bound": "mfma", "achieved": res["infer_tflops"], "peak": MFMA_BF16_PEAK_TFLOPS,
-                         "unit": "TFLOP/s", "frac": round(res["infer_tflops"] / MFMA_BF16_PEAK_TFLOPS, 4)}
+                         "unit": "TFLOP/s", "frac": round(res["infer_tflops"] / MFMA_BF16_PEAK_TFLOPS, 4),
+                         "frac_of_measured": round(res["infer_tflops"] / peaks["mfma_bf16_tflops"], 4)}
     return res
 
 
 def inverse(ctx, steps=200):
     """Inverse rendering: 200 Adam steps on 8 material albedos through the
-    path-replay adjoint, icosphere scene 512 x 512, 32 spp, depth 6."""
+    path-replay adjoint, icosphere scene 512 x 512, 32 spp, depth 6; then
+    the adjoint's time per call in each form (replay, tape)."""
     import torch
 
     from mcrt_amd import diff
@@ -284,47 +614,101 @@ def inverse(ctx, steps=200):
     ctx.upload_scene(inv.work.buffers)
     w = inv.work
     target_spp = 8 * w.spp
+    ctx.set_diff_mode("replay")
     diff.optimize(ctx, inv, steps=2, target_spp=target_spp)  # warm-up
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     _, hist = diff.optimize(ctx, inv, steps=steps, target_spp=target_spp)
     torch.cuda.synchronize()
     el = time.perf_counter() - t0
+    forms = {}
+    for mode in ("replay", "tape"):
+        ctx.set_diff_mode(mode)
+        diff.optimize(ctx, inv, steps=2, target_spp=w.spp)
+        torch.cuda.synchronize()
+        ctx.set_profiling(True)
+        ctx.reset_stats()
+        diff.optimize(ctx, inv, steps=20, target_spp=w.spp)
+        torch.cuda.synchronize()
+        fms, fn = ctx.kernel_stats("diff_primal")
+        ams, an = ctx.kernel_stats("diff_adjoint")
+        ctx.set_profiling(False)
+        tape_bytes = 0
+        if mode == "tape":
+            from mcrt_amd._lib import diff_tape_bytes
+
+            tape_bytes = diff_tape_bytes(w.width, w.height, w.spp, w.depth)
+        forms[mode] = {"adjoint_ms": round(ams / max(an, 1), 3), "primal_ms": round(fms / max(fn, 1), 3),
+                       "tape_bytes": tape_bytes}
+    ctx.set_diff_mode("replay")
     return {"workload": f"{w.name} {w.width}x{w.height} {w.spp} spp depth {w.depth}, {len(inv.init)} "
                         f"materials x (RGB albedo, emission); target {target_spp} spp, independent frames; "
                         "decorrelated primal / replay frames per step",
+            "adjoint": "replay (re-trace, no tape)",
             "steps": steps, "total_s": round(el, 3), "ms_per_step": round(el / steps * 1e3, 3),
-            # samples of the primal render plus samples of the adjoint (each traced and replayed)
+            # samples of the primal render plus samples of the adjoint (each traced once)
             "msamples_fwd_plus_adj_per_s": round(2 * w.width * w.height * w.spp * steps / el / 1e6, 1),
             "loss_first": round(hist[0][0], 6), "loss_last": round(hist[-1][0], 6),
             "albedo_err_first": {"max": round(hist[0][1], 4), "mean": round(hist[0][2], 4)},
-            "albedo_err_last": {"max": round(hist[-1][1], 4), "mean": round(hist[-1][2], 4)}}
+            "albedo_err_last": {"max": round(hist[-1][1], 4), "mean": round(hist[-1][2], 4)},
+            "adjoint_forms": forms}
+
+
+def _cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def cpu_baseline(w, seconds):
     """The CPU oracle (FP32 C restatement of the reference shader, pthreads)
-    rendering whole frames of the same workload until ~`seconds` elapse."""
+    rendering whole frames of the same workload: median of 3 timed runs of
+    equal frame counts sized to ~`seconds` in total; the host SAH build of
+    the same mesh is timed on its own."""
     import oracle
+    from mcrt_amd import build_mesh_bvh
 
-    threads = min(os.cpu_count() or 1, 16)
+    nproc = os.cpu_count() or 1
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = nproc
+    # One GPU's share of a GPU box's host is 16 cores whatever the machine
+    # reports; a host with fewer available uses all of them.
+    threads = max(1, min(avail, 16))
     sc = oracle.Scene(w.buffers)
-    frames = 0
     t0 = time.perf_counter()
-    acc = None
-    while True:
-        acc, _ = oracle.render(sc, w.camera, w.width, w.height, 1, frames, w.depth, accum_in=acc,
-                               threads=threads)
-        frames += 1
-        el = time.perf_counter() - t0
-        if el >= seconds or frames >= w.spp:
-            break
+    oracle.render(sc, w.camera, w.width, w.height, 1, 0, w.depth, threads=threads)
+    t1 = time.perf_counter() - t0
+    frames = max(1, int(round(seconds / 3.0 / t1)))
+    frames = min(frames, w.spp)
+    runs = []
+    for r in range(3):
+        t0 = time.perf_counter()
+        oracle.render(sc, w.camera, w.width, w.height, frames, 1 + r * frames, w.depth, threads=threads)
+        runs.append(time.perf_counter() - t0)
+    el = statistics.median(runs)
     value = w.width * w.height * frames / el / 1e6
+    t0 = time.perf_counter()
+    if w.loaded is not None and len(w.loaded.mesh_triangles):
+        build_mesh_bvh(w.loaded.mesh_vertices, w.loaded.mesh_triangles)
+    bvh_s = time.perf_counter() - t0
     return {
         "value": round(value, 4),
         "unit": "Msamples/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"{frames} of {w.spp} frames of {w.name} at {w.width}x{w.height}, depth {w.depth}",
+        "sample": f"median of 3 runs of {frames} frame(s) of {w.name} at {w.width}x{w.height}, depth {w.depth} "
+                  f"(render only; runs {', '.join(f'{x:.2f}' for x in runs)} s)",
+        "host": {"nproc": nproc, "available_cores": avail, "cpu_model": _cpu_model(),
+                 "threads_note": "16 = one GPU's share of the GPU box's host cores"},
+        "bvh_build_s": round(bvh_s, 3),
+        "bvh_build": f"host C++ binned SAH of the {len(w.buffers.triangles)}-triangle mesh, single thread",
+        "label": "CPU restatement of the reference shader, not the reference binary",
     }
 
 

@@ -406,6 +406,13 @@ int mcrt_reset_stats(mcrt_ctx *ctx);
  * samples produced.  Synchronises the context's last stream. */
 int mcrt_render_counters(mcrt_ctx *ctx, uint64_t *segments, uint64_t *samples);
 
+/* Peak probes for roofline fractions, no counterpart in the reference: a
+ * float4 copy between two buffers of copy_bytes each (GB/s of bytes read +
+ * written) and a loop of register-operand v_mfma_f32_16x16x32_bf16 on every
+ * SIMD (dense BF16 TFLOP/s), both timed with HIP events, best of several
+ * runs.  Allocates 2 x copy_bytes on `device` for the call; blocks. */
+int mcrt_measure_peaks(int device, uint64_t copy_bytes, double *copy_gbs, double *mfma_bf16_tflops);
+
 #ifdef __cplusplus
 }
 #endif

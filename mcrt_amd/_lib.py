@@ -83,6 +83,7 @@ def load() -> C.CDLL:
         "mcrt_kernel_stats": (C.c_int, [_vp, C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_uint64)]),
         "mcrt_reset_stats": (C.c_int, [_vp]),
         "mcrt_render_counters": (C.c_int, [_vp, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
+        "mcrt_measure_peaks": (C.c_int, [C.c_int, C.c_uint64, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
         "mcrt_diff_render": (C.c_int, [_vp, _vp, _u32, _u32, _u32, _u32, _u32, _vp, _u32, _vp, _vp]),
         "mcrt_diff_render_dev": (C.c_int, [_vp, _vp, _u32, _u32, _u32, _u32, _u32, _vp, _u32, _vp, _vp, _vp]),
         "mcrt_diff_adjoint": (C.c_int, [_vp, _vp, _u32, _u32, _u32, _u32, _u32, _vp, _u32, _vp, _vp, _vp]),
@@ -119,12 +120,32 @@ def exported_symbols() -> list:
         "mcrt_render_dev", "mcrt_denoise_bilateral", "mcrt_denoise_bilateral_dev",
         "mcrt_trace_closest", "mcrt_trace_closest_dev", "mcrt_trace_closest_noalloc_dev", "mcrt_build_mesh_bvh",
         "mcrt_build_sphere_bvh", "mcrt_validate_bvh", "mcrt_set_profiling", "mcrt_set_nee", "mcrt_kernel_stats", "mcrt_reset_stats",
-        "mcrt_render_counters", "mcrt_mesh_voxel_chunks", "mcrt_diff_render", "mcrt_diff_render_dev",
+        "mcrt_render_counters", "mcrt_measure_peaks", "mcrt_mesh_voxel_chunks", "mcrt_diff_render", "mcrt_diff_render_dev",
         "mcrt_diff_adjoint", "mcrt_diff_adjoint_dev", "mcrt_diff_set_mode", "mcrt_render_aux_dev", "mcrt_denoiser_create",
         "mcrt_denoiser_destroy", "mcrt_denoiser_param_count", "mcrt_denoiser_get_weights",
         "mcrt_denoiser_set_weights", "mcrt_denoiser_infer_dev", "mcrt_denoiser_train_step_dev",
         "mcrt_denoiser_gradient", "mcrt_denoiser_set_loss",
     ]
+
+
+def measure_peaks(device: int = 0, copy_bytes: int = 1 << 30):
+    """(copy GB/s, dense BF16 MFMA TFLOP/s) measured on `device`."""
+    lib = load()
+    g, t = C.c_double(0.0), C.c_double(0.0)
+    rc = lib.mcrt_measure_peaks(device, copy_bytes, C.byref(g), C.byref(t))
+    if rc != MCRT_OK:
+        raise McrtError(rc, "mcrt_measure_peaks failed")
+    return g.value, t.value
+
+
+def diff_tape_bytes(width: int, height: int, spp: int, max_depth: int) -> int:
+    """Device memory of the tape adjoint (MCRT_DIFF_TAPE) for a render size:
+    48 B per vertex for every thread of its grid (frame groups until about
+    2^22 threads, 256-thread blocks)."""
+    wh = width * height
+    groups = max(1, min(spp, (1 << 22) // wh))
+    grid = (wh * groups + 255) // 256
+    return grid * 256 * max_depth * 48
 
 
 def validate_bvh(nodes: np.ndarray, n_prims: int) -> None:

@@ -17,7 +17,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libmcrt_hip.so")
 SOURCES = ["capi.hip", "render_kernels.hip", "denoise_kernels.hip", "learned_kernels.hip",
-           "adjoint_kernels.hip", "bvh_build.cpp", "voxel_mesh.cpp"]
+           "adjoint_kernels.hip", "probe_kernels.hip", "bvh_build.cpp", "voxel_mesh.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
     "-O3", "-std=c++17", "--offload-arch=gfx950", "-fPIC", "-shared", "-ffp-contract=off",
