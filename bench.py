@@ -134,7 +134,7 @@ def probe(cache_dir):
 
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
-    measure_peaks(0, COPY_PROBE_BYTES)
+    measure_peaks(0, COPY_PROBE_BYTES)  # its copy dispatches calibrate FETCH_SIZE / WRITE_SIZE
     with Context(0) as ctx:
         for name in ("c3", "c2"):
             w = load_workload(os.path.join(cache_dir, f"{name}.npz"))
@@ -371,12 +371,14 @@ def measured_peaks():
     from mcrt_amd._lib import measure_peaks
 
     log("peaks: copy and MFMA probes")
-    gbs, tflops = measure_peaks(0, COPY_PROBE_BYTES)
+    gbs, tflops, tflops16 = measure_peaks(0, COPY_PROBE_BYTES)
     return {"hbm_copy_gbs": round(gbs, 1), "hbm_copy_frac_of_published": round(gbs / HBM_PEAK_GBS, 4),
             "mfma_bf16_tflops": round(tflops, 1),
             "mfma_bf16_frac_of_published": round(tflops / MFMA_BF16_PEAK_TFLOPS, 4),
-            "method": "mcrt_measure_peaks: float4 copy of 2 x 1 GiB, register-operand v_mfma_f32_16x16x32_bf16 "
-                      "loop on every SIMD; HIP events, best of several runs"}
+            "mfma_bf16_16x16x32_tflops": round(tflops16, 1),
+            "method": "mcrt_measure_peaks: one-float4-per-thread copy of 1 GiB (read + write counted), "
+                      "register-operand MFMA loops on every SIMD (v_mfma_f32_32x32x16_bf16; "
+                      "v_mfma_f32_16x16x32_bf16, the convolutions' shape); HIP events, best of several runs"}
 
 
 def roofline(ctx, w, step, segments, counters, peaks):
@@ -597,7 +599,8 @@ def denoise_eval(ctx, peaks):
         dn_nee.close()
     res["infer_mfma"] = {"bound": "mfma", "achieved": res["infer_tflops"], "peak": MFMA_BF16_PEAK_TFLOPS,
                          "unit": "TFLOP/s", "frac": round(res["infer_tflops"] / MFMA_BF16_PEAK_TFLOPS, 4),
-                         "frac_of_measured": round(res["infer_tflops"] / peaks["mfma_bf16_tflops"], 4)}
+                         "frac_of_measured": round(res["infer_tflops"] / peaks["mfma_bf16_tflops"], 4),
+                         "frac_of_measured_16x16x32": round(res["infer_tflops"] / peaks["mfma_bf16_16x16x32_tflops"], 4)}
     return res
 
 

@@ -83,7 +83,8 @@ def load() -> C.CDLL:
         "mcrt_kernel_stats": (C.c_int, [_vp, C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_uint64)]),
         "mcrt_reset_stats": (C.c_int, [_vp]),
         "mcrt_render_counters": (C.c_int, [_vp, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
-        "mcrt_measure_peaks": (C.c_int, [C.c_int, C.c_uint64, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+        "mcrt_measure_peaks": (C.c_int, [C.c_int, C.c_uint64, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                         C.POINTER(C.c_double)]),
         "mcrt_diff_render": (C.c_int, [_vp, _vp, _u32, _u32, _u32, _u32, _u32, _vp, _u32, _vp, _vp]),
         "mcrt_diff_render_dev": (C.c_int, [_vp, _vp, _u32, _u32, _u32, _u32, _u32, _vp, _u32, _vp, _vp, _vp]),
         "mcrt_diff_adjoint": (C.c_int, [_vp, _vp, _u32, _u32, _u32, _u32, _u32, _vp, _u32, _vp, _vp, _vp]),
@@ -129,13 +130,14 @@ def exported_symbols() -> list:
 
 
 def measure_peaks(device: int = 0, copy_bytes: int = 1 << 30):
-    """(copy GB/s, dense BF16 MFMA TFLOP/s) measured on `device`."""
+    """(copy GB/s, BF16 MFMA TFLOP/s with 32x32x16, with 16x16x32) measured
+    on `device`."""
     lib = load()
-    g, t = C.c_double(0.0), C.c_double(0.0)
-    rc = lib.mcrt_measure_peaks(device, copy_bytes, C.byref(g), C.byref(t))
+    g, t, t16 = C.c_double(0.0), C.c_double(0.0), C.c_double(0.0)
+    rc = lib.mcrt_measure_peaks(device, copy_bytes, C.byref(g), C.byref(t), C.byref(t16))
     if rc != MCRT_OK:
         raise McrtError(rc, "mcrt_measure_peaks failed")
-    return g.value, t.value
+    return g.value, t.value, t16.value
 
 
 def diff_tape_bytes(width: int, height: int, spp: int, max_depth: int) -> int:
