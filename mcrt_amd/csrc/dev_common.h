@@ -244,20 +244,23 @@ struct TStack {
     uint32_t *lds;
     uint32_t *ovf;
     uint32_t ovf_stride, tid, gtid, block;
+    // Reads and writes go through explicitly LDS- and global-qualified
+    // pointers.  Through plain pointers the compiler merges the LDS and the
+    // overflow access into one flat access, which routes every LDS push or pop
+    // through the vector-memory path (its address unit is the busiest unit of
+    // the traversal) and makes each node step wait on both counters.
     __device__ __forceinline__ void set(uint32_t i, uint32_t v, float te) const {
         if (i < kShort) {
-            lds[i * block + tid] = v;
-            if (kCull) lds[(kShort + i) * block + tid] = __float_as_uint(te);
+            __attribute__((address_space(3))) uint32_t *l = (__attribute__((address_space(3))) uint32_t *)lds;
+            l[i * block + tid] = v;
+            if (kCull) l[(kShort + i) * block + tid] = __float_as_uint(te);
         } else {
+            __attribute__((address_space(1))) uint32_t *g = (__attribute__((address_space(1))) uint32_t *)ovf;
             const size_t e = (size_t)(i - kShort) * kWords * ovf_stride + gtid;
-            ovf[e] = v;
-            if (kCull) ovf[e + ovf_stride] = __float_as_uint(te);
+            g[e] = v;
+            if (kCull) g[e + ovf_stride] = __float_as_uint(te);
         }
     }
-    // The reads go through explicitly LDS- and global-qualified pointers.
-    // Through plain pointers the compiler merges them into one flat load,
-    // which routes every LDS pop through the vector-memory path and makes each
-    // node step wait on both counters.
     __device__ __forceinline__ uint32_t get(uint32_t i, float &te) const {
         if (i < kShort) {
             const __attribute__((address_space(3))) uint32_t *l = (const __attribute__((address_space(3))) uint32_t *)lds;
@@ -306,8 +309,12 @@ __device__ __forceinline__ void node_step(const float4 *nodes, V3 o, V3 inv, flo
     slab2(o, inv, q0.x, q0.y, q0.z, q0.w, q2.x, q2.y, te0, tx0);
     slab2(o, inv, q1.x, q1.y, q1.z, q1.w, q2.z, q2.w, te1, tx1);
     const float lim = best_t * kCullSlack;
-    bool h0 = tx0 > fmaxf(te0, t_min) && te0 < t_max && !(te0 > lim);
-    bool h1 = tx1 > fmaxf(te1, t_min) && te1 < t_max && !(te1 > lim);
+    // te < t_max (the reference test) and !(te > lim) (the cull) as one
+    // compare: te < min(t_max, next float above lim) holds exactly when both
+    // do, and is false for a NaN te, which the reference test rejects too.
+    const float bound = fminf(t_max, __uint_as_float(__float_as_uint(lim) + 1u));
+    bool h0 = tx0 > fmaxf(te0, t_min) && te0 < bound;
+    bool h1 = tx1 > fmaxf(te1, t_min) && te1 < bound;
     uint32_t c0 = __float_as_uint(q3.x), c1 = __float_as_uint(q3.y);
     if (h0 && h1) {
         uint32_t nearc = c0, farc = c1;
