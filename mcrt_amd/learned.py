@@ -41,6 +41,9 @@ PKG_DIR = os.path.dirname(os.path.abspath(__file__))
 WEIGHTS_PATH = os.path.join(PKG_DIR, "weights", "denoiser_c4.npy")
 WEIGHTS_NEE_PATH = os.path.join(PKG_DIR, "weights", "denoiser_c4_nee.npy")  # trained on NEE renders
 REFERENCE_CROPS = os.path.join(os.path.dirname(PKG_DIR), "tests", "golden", "c4_reference_crops.npz")
+# Full-frame reference of the same frame and frames (GPU, 65,536 spp, float16;
+# tests/golden/make_c4_full_reference.py), checked against the windows.
+REFERENCE_FULL = os.path.join(os.path.dirname(PKG_DIR), "tests", "golden", "c4_full_reference.npz")
 EVAL_W, EVAL_H, EVAL_DEPTH = 1920, 1080, 8
 INFER_FLOP_PER_PIXEL = 81216  # 2 * 9 * (10*32 + 4*32*32 + 32*3) multiply-adds of the six convs
 
@@ -276,6 +279,14 @@ def load_reference_crops(path: str = REFERENCE_CROPS):
     return d["crops"].astype(np.float64), [tuple(int(v) for v in o) for o in d["origins"]], int(d["spp"])
 
 
+def load_full_reference(path: str = REFERENCE_FULL):
+    """(H, W, 3) float32 full-frame reference, or None without the fixture."""
+    if not os.path.exists(path):
+        return None
+    d = np.load(path, allow_pickle=False)
+    return d["image"].astype(np.float32)
+
+
 def crops_rel_mse(img: np.ndarray, crops: np.ndarray, origins) -> float:
     c = crops.shape[1]
     x = np.stack([img[y0:y0 + c, x0:x0 + c, :3] for x0, y0 in origins])
@@ -283,21 +294,23 @@ def crops_rel_mse(img: np.ndarray, crops: np.ndarray, origins) -> float:
 
 
 def time_to_rel_mse(ctx: Context, wl, method: str, spps, crops, origins, dn: Denoiser = None,
-                    threshold: float = 1e-2, reps: int = 3, log=None):
+                    threshold: float = 1e-2, reps: int = 3, log=None, full=None):
     """Sweep spp upward; at each, time render (+ denoise) over `reps` runs
     (median, inputs resident, scene uploaded) and take relMSE on the
-    reference windows.  Stops at the first spp at or below `threshold`.
-    method: "none", "bilateral" or "learned", optionally prefixed "nee_"
-    (or "nee" alone) for renders with next-event estimation."""
+    reference windows and, with `full` (the full-frame reference), on every
+    pixel.  Stops at the first spp at or below `threshold` on the full frame
+    (on the windows without it).  method: "none", "bilateral" or "learned",
+    optionally prefixed "nee_" (or "nee" alone) for renders with next-event
+    estimation."""
     ctx.set_nee(method.startswith("nee"))
     try:
         return _time_to_rel_mse(ctx, wl, method.replace("nee_", "").replace("nee", "none"), spps, crops,
-                                origins, dn, threshold, reps, log, method)
+                                origins, dn, threshold, reps, log, method, full)
     finally:
         ctx.set_nee(False)
 
 
-def _time_to_rel_mse(ctx, wl, method, spps, crops, origins, dn, threshold, reps, log, label):
+def _time_to_rel_mse(ctx, wl, method, spps, crops, origins, dn, threshold, reps, log, label, full=None):
     import torch
 
     dev = torch.device("cuda:0")
@@ -325,11 +338,16 @@ def _time_to_rel_mse(ctx, wl, method, spps, crops, origins, dn, threshold, reps,
         # Above 128 spp one run lasts ~0.5 s or more and varies little.
         times = [run(spp) for _ in range(reps if spp <= 128 else 1)]
         img = (accum if method == "none" else out).cpu().numpy()
-        err = crops_rel_mse(img, crops, origins)
-        row = {"spp": spp, "ms": round(float(np.median(times)) * 1e3, 3), "relmse": round(err, 6)}
+        err_win = crops_rel_mse(img, crops, origins)
+        row = {"spp": spp, "ms": round(float(np.median(times)) * 1e3, 3), "relmse_windows": round(err_win, 6)}
+        err = err_win
+        if full is not None:
+            err = rel_mse(img, full)
+            row["relmse_full"] = round(err, 6)
+        row["relmse"] = round(err, 6)
         rows.append(row)
         if log:
-            log(f"{label} spp {spp}: {row['ms']:.2f} ms relMSE {err:.4e}")
+            log(f"{label} spp {spp}: {row['ms']:.2f} ms relMSE {err:.4e} (windows {err_win:.4e})")
         if err <= threshold:
             hit = row
             break
@@ -368,19 +386,24 @@ def evaluate(ctx: Context, dn: Denoiser, reps: int = 3, log=None, dn_nee: Denois
     """The sweep for every method; `dn_nee` (weights trained on renders with
     next-event estimation) denoises the NEE renders when given."""
     crops, origins, ref_spp = load_reference_crops()
+    full = load_full_reference()
     wl = eval_workload()
     ctx.upload_scene(wl.buffers)
     # Steps of at most ~1/3 so that neither filter is charged for a coarse grid.
     spps = [1, 2, 3, 4, 5, 6, 7, 8, 10, 12, 14, 16, 20, 24, 28, 32, 40, 48, 56, 64, 80, 96, 128, 160, 192,
             256, 320, 384, 512, 640, 768, 1024, 1280, 1536, 2048]
     res = {"frame": f"icosphere_c2 {EVAL_W}x{EVAL_H} depth {EVAL_DEPTH}",
-           "reference": f"{len(origins)} windows {crops.shape[1]}x{crops.shape[2]}, {ref_spp} spp FP64 oracle",
+           "reference": (f"every pixel: GPU {ref_spp}-spp reference of the oracle's frames, checked against "
+                         f"{len(origins)} FP64 oracle windows" if full is not None else
+                         f"{len(origins)} windows {crops.shape[1]}x{crops.shape[2]}, {ref_spp} spp FP64 oracle"),
+           "relmse_over": "full frame" if full is not None else "windows",
            "threshold": 1e-2}
     # The metric's comparison (plain renders), then the same with next-event
     # estimation.
     for method in ("learned", "bilateral", "none", "nee_learned", "nee_bilateral", "nee"):
         net = dn_nee if (method == "nee_learned" and dn_nee is not None) else dn
-        res[method] = time_to_rel_mse(ctx, wl, method, spps, crops, origins, dn=net, reps=reps, log=log)
+        res[method] = time_to_rel_mse(ctx, wl, method, spps, crops, origins, dn=net, reps=reps, log=log,
+                                      full=full)
     res["nee_learned_weights"] = "trained on NEE renders" if dn_nee is not None else "trained on plain renders"
     t_inf = time_infer(dn, EVAL_W, EVAL_H)
     res["infer_ms"] = round(t_inf * 1e3, 3)

@@ -53,3 +53,20 @@ def test_rel_mse_closed_forms():
     # Alpha channel is ignored.
     a = np.concatenate([y, np.full((4, 4, 1), 7.0)], -1)
     assert learned.rel_mse(a, np.concatenate([y, np.zeros((4, 4, 1))], -1)) == 0.0
+
+
+def test_full_frame_reference_matches_the_oracle_windows():
+    """The GPU full-frame reference (65,536 spp of the same frames, float16)
+    agrees with the FP64 oracle windows up to float16 rounding and rare path
+    divergences: it was rendered from the same PCG frames."""
+    from mcrt_amd import learned
+
+    full = learned.load_full_reference()
+    assert full is not None and full.shape == (1080, 1920, 3)
+    assert np.all(np.isfinite(full)) and full.min() >= 0.0
+    crops, origins, spp = learned.load_reference_crops()
+    z = np.load(learned.REFERENCE_FULL, allow_pickle=False)
+    assert int(z["spp"]) == spp == 65536 and int(z["first_frame"]) == 1 << 24
+    assert learned.crops_rel_mse(full, crops, origins) < 2e-6
+    # as rendered, before the float16 store
+    assert float(z["relmse_vs_oracle_windows"]) < 1e-9
