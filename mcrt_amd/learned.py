@@ -435,6 +435,8 @@ def _main(argv=None):
     e.add_argument("--weights", default=WEIGHTS_PATH)
     e.add_argument("--weights-nee", default=WEIGHTS_NEE_PATH)
     e.add_argument("--reps", type=int, default=3)
+    ib = sub.add_parser("infer-bench", help="time inference on a 1920 x 1080 frame (committed weights)")
+    ib.add_argument("--reps", type=int, default=20)
     a = ap.parse_args(argv)
 
     def log(msg):
@@ -461,6 +463,11 @@ def _main(argv=None):
                                    "loss_first": losses[0], "loss_last": losses[-1],
                                    "pairs_s": round(t_pairs, 1), "train_s": round(t_train, 1)})
             log(f"weights -> {out}")
+        elif a.cmd == "infer-bench":
+            dn = load_trained(ctx)
+            t = time_infer(dn, EVAL_W, EVAL_H, reps=a.reps)
+            print(json.dumps({"frame": f"{EVAL_W}x{EVAL_H}", "infer_ms": round(t * 1e3, 4),
+                              "tflops": round(INFER_FLOP_PER_PIXEL * EVAL_W * EVAL_H / t / 1e12, 1)}), flush=True)
         else:
             dn = load_trained(ctx, a.weights)
             dn_nee = load_trained(ctx, a.weights_nee) if os.path.exists(a.weights_nee) else None

@@ -94,6 +94,44 @@ def test_inference_matches_fp32_reference(ctx, batch):
     assert (got_log - full).abs().max().item() < 0.1
 
 
+def test_inference_at_ragged_sizes_matches_reference_and_training_forward(ctx):
+    """At sizes that are not multiples of the 16 x 16 tile (partial tiles,
+    image borders inside a tile's halo) inference matches the FP32
+    reference, and its output gives the loss the training forward pass
+    reports for the same input."""
+    import torch
+
+    dev = torch.device("cuda:0")
+    rng = np.random.default_rng(5)
+    for (B, H, W) in ((1, 37, 53), (3, 17, 16), (2, 5, 70)):
+        noisy = torch.tensor(rng.gamma(1.0, 0.5, (B, H, W, 4)).astype(np.float32), device=dev)
+        albedo = torch.tensor(rng.random((B, H, W, 4)).astype(np.float32), device=dev)
+        gbuf = torch.tensor(rng.normal(0, 1, (B, H, W, 4)).astype(np.float32), device=dev)
+        gbuf[..., 3] = torch.tensor(rng.uniform(0.5, 30, (B, H, W)).astype(np.float32), device=dev)
+        ref = torch.tensor(rng.gamma(1.0, 0.5, (B, H, W, 4)).astype(np.float32), device=dev)
+        dn = learned.Denoiser(ctx, seed=11)
+        w = dn.get_weights()
+        last = sum(co * ci * 9 + co for co, ci in learned.layer_shapes()[:5])
+        w[last:] = rng.normal(0, 0.05, len(w) - last).astype(np.float32)
+        dn.set_weights(w)
+        out = torch.empty_like(noisy)
+        dn.infer(noisy, albedo, gbuf, out)
+        torch.cuda.synchronize()
+        ref_log = torch_reference(_weights_torch(dn), noisy, albedo, gbuf)
+        got_log = torch.log1p(out[..., :3])
+        err = (got_log - ref_log).abs().max().item()
+        print(f"{B}x{H}x{W}: max |log out - log ref| = {err:.2e}")
+        # Random inputs push some BF16 activations across a rounding boundary
+        # between the FP32 accumulation orders of F.conv2d and the MFMAs; the
+        # tolerance is wider than for rendered inputs.
+        assert err < 1e-2
+        # the training forward pass's loss on the same input
+        _, loss_layers = dn.gradient(noisy, albedo, gbuf, ref)
+        loss_fused = torch.mean((got_log - torch.log1p(torch.clamp(ref[..., :3], min=0.0))) ** 2).item()
+        assert abs(loss_fused - loss_layers) <= 2e-4 * abs(loss_layers), (loss_fused, loss_layers)
+        dn.close()
+
+
 @pytest.mark.parametrize("kind", ["log_l2", "rel_mse"])
 def test_gradient_matches_autograd(ctx, batch, kind):
     import torch
