@@ -1500,9 +1500,30 @@ extern "C" int mcrt_denoiser_infer_dev(mcrt_denoiser *d, uint32_t batch, uint32_
     if (!noisy || !albedo || !gbuf || !out) return fail(d->c, MCRT_ERR_BAD_ARG, "null image pointer");
     hipStream_t s = (hipStream_t)stream;
     d->c->last_stream = s;
-    if ((rc = dn_forward(d, (int)batch, (int)width, (int)height, noisy, albedo, gbuf, s))) return rc;
-    HIP_TRY(d->c, launch_unpack((const float4 *)d->out_log.p, (float4 *)out,
-                                (size_t)batch * width * height, s));
+    // The training forward pass without its log-space output: the last
+    // layer writes the denoised image.
+    PackArgs pa{(const float4 *)noisy, (const float4 *)albedo, (const float4 *)gbuf, (__bf16 *)d->act[0].p,
+                (int)batch, (int)height, (int)width};
+    HIP_TRY(d->c, launch_pack(pa, s));
+    for (int l = 0; l < kDenoiseLayers; ++l) {
+        ConvArgs ca{};
+        ca.in = (const __bf16 *)d->act[l].p;
+        ca.noisy = (const float4 *)noisy;
+        ca.w = (const __bf16 *)d->wf[l].p;
+        ca.bias = (const float *)d->w[l].p + 9 * 32 * 32;
+        ca.B = (int)batch;
+        ca.H = (int)height;
+        ca.W = (int)width;
+        if (l + 1 < kDenoiseLayers) {
+            ca.out = (__bf16 *)d->act[l + 1].p;
+            ca.mode = 0;
+        } else {
+            ca.out_final = (float4 *)out;
+            ca.mode = 2;
+        }
+        Timed t(d->c, "denoise_conv", s);
+        HIP_TRY(d->c, launch_conv3x3(ca, s));
+    }
     return MCRT_OK;
 }
 

@@ -165,6 +165,9 @@ struct ConvArgs {
     __bf16 *out;         // modes 0, 1
     float4 *out_log;     // mode 2
     const float4 *noisy; // mode 2 residual source
+    // Inference only (mode 2): write the denoised image expm1(out_log),
+    // alpha 1, instead of out_log.
+    float4 *out_final;
     int B, H, W;
     int mode;            // 0: ReLU, BF16 out; 1: linear, BF16 out; 2: final layer
 };

@@ -58,21 +58,26 @@ __device__ __forceinline__ uint4 apply_mask(uint4 v, uint4 m) {
 // ---------------------------------------------------------------------------
 // Input packing: renders -> BF16 NHWC32 network input
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_pack(PackArgs a) {
-    const size_t n = (size_t)a.B * a.H * a.W;
-    const size_t p = (size_t)blockIdx.x * 256 + threadIdx.x;
-    if (p >= n) return;
-    const float4 L = a.noisy[p], al = a.albedo[p], g = a.gbuf[p];
+// The network's 10 input channels of one pixel (channels 0-7 in lo, 8-9 and
+// zeros in hi): log1p radiance, albedo, normal, scaled log depth.
+__device__ __forceinline__ void input_channels(float4 L, float4 al, float4 g, bf16x8 &lo, bf16x8 &hi) {
     const float d = fminf(fmaxf(g.w, 0.0f), 1e4f);
     float v[16] = {log1pf(fmaxf(L.x, 0.0f)), log1pf(fmaxf(L.y, 0.0f)), log1pf(fmaxf(L.z, 0.0f)),
                    al.x, al.y, al.z, g.x, g.y, g.z, log1pf(d) * (1.0f / 9.21044f),
                    0, 0, 0, 0, 0, 0};
-    bf16x8 lo, hi;
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
         lo[k] = f2b(v[k]);
         hi[k] = f2b(v[8 + k]);
     }
+}
+
+__global__ __launch_bounds__(256) void k_pack(PackArgs a) {
+    const size_t n = (size_t)a.B * a.H * a.W;
+    const size_t p = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (p >= n) return;
+    bf16x8 lo, hi;
+    input_channels(a.noisy[p], a.albedo[p], a.gbuf[p], lo, hi);
     bf16x8 z = {};
     bf16x8 *o = reinterpret_cast<bf16x8 *>(a.out + p * kC);
     o[0] = lo;
@@ -99,7 +104,11 @@ __global__ __launch_bounds__(256) void k_pack(PackArgs a) {
 // (4 waves).  Holding one half of the weights per wave (36 VGPRs of B
 // fragments instead of 72) doubles the resident waves, which hide the
 // tile loads of the other blocks.
-template <bool kFinal>
+// kEnd (inference, last layer only): write the denoised image
+// (ConvArgs::out_final) instead of the log-space output, so no unpack pass
+// follows; a separate instantiation, so the training kernel carries none of
+// it.
+template <bool kFinal, bool kEnd = false>
 __global__ __launch_bounds__(kFinal ? 256 : 512) void k_conv3x3(ConvArgs a) {
     constexpr int kThreads = kFinal ? 256 : 512;
     __shared__ __align__(16) bf16 s_in[kH * kH * kPS];
@@ -133,7 +142,8 @@ __global__ __launch_bounds__(kFinal ? 256 : 512) void k_conv3x3(ConvArgs a) {
             pre[k] = make_uint4(0, 0, 0, 0);
             pmask[k] = make_uint4(0, 0, 0, 0);
             if (i < kH * kH * 4 && gy >= 0 && gy < a.H && gx >= 0 && gx < a.W) {
-                const size_t off = (((size_t)b * a.H + gy) * a.W + gx) * kC + q * 8;
+                const size_t px = ((size_t)b * a.H + gy) * a.W + gx;
+                const size_t off = px * kC + q * 8;
                 pre[k] = *reinterpret_cast<const uint4 *>(a.in + off);
                 if (a.mask) pmask[k] = *reinterpret_cast<const uint4 *>(a.mask + off);
             }
@@ -190,9 +200,13 @@ __global__ __launch_bounds__(kFinal ? 256 : 512) void k_conv3x3(ConvArgs a) {
                 if (gy < a.H && gx < a.W) {
                     const size_t p = ((size_t)b * a.H + gy) * a.W + gx;
                     const float4 nz = a.noisy[p];
-                    a.out_log[p] = make_float4(log1pf(fmaxf(nz.x, 0.0f)) + s_f[px * 3 + 0],
-                                               log1pf(fmaxf(nz.y, 0.0f)) + s_f[px * 3 + 1],
-                                               log1pf(fmaxf(nz.z, 0.0f)) + s_f[px * 3 + 2], 0.0f);
+                    const float4 o = make_float4(log1pf(fmaxf(nz.x, 0.0f)) + s_f[px * 3 + 0],
+                                                 log1pf(fmaxf(nz.y, 0.0f)) + s_f[px * 3 + 1],
+                                                 log1pf(fmaxf(nz.z, 0.0f)) + s_f[px * 3 + 2], 0.0f);
+                    if (kEnd)  // inference: the denoised image, as k_unpack
+                        a.out_final[p] = make_float4(expm1f(o.x), expm1f(o.y), expm1f(o.z), 1.0f);
+                    else
+                        a.out_log[p] = o;
                 }
             }
             continue;
@@ -447,7 +461,7 @@ uint32_t conv_grid(size_t tiles) {
         int dev = 0, cus = 0, per_cu = 0;
         if (hipGetDevice(&dev) != hipSuccess ||
             hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-            hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void *>(&k_conv3x3<kFinal>),
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void *>(&k_conv3x3<kFinal, false>),
                                                          kFinal ? 256 : 512, 0) != hipSuccess ||
             cus < 1 || per_cu < 1)
             resident = 1024;
@@ -459,10 +473,12 @@ uint32_t conv_grid(size_t tiles) {
 
 hipError_t launch_conv3x3(const ConvArgs &a, hipStream_t s) {
     const size_t tiles = (size_t)((a.W + kT - 1) / kT) * ((a.H + kT - 1) / kT) * a.B;
-    if (a.mode == 2)
-        hipLaunchKernelGGL(k_conv3x3<true>, dim3(conv_grid<true>(tiles)), dim3(256), 0, s, a);
+    if (a.mode == 2 && a.out_final)
+        hipLaunchKernelGGL((k_conv3x3<true, true>), dim3(conv_grid<true>(tiles)), dim3(256), 0, s, a);
+    else if (a.mode == 2)
+        hipLaunchKernelGGL((k_conv3x3<true, false>), dim3(conv_grid<true>(tiles)), dim3(256), 0, s, a);
     else
-        hipLaunchKernelGGL(k_conv3x3<false>, dim3(conv_grid<false>(tiles)), dim3(512), 0, s, a);
+        hipLaunchKernelGGL((k_conv3x3<false, false>), dim3(conv_grid<false>(tiles)), dim3(512), 0, s, a);
     return hipGetLastError();
 }
 
