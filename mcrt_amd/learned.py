@@ -154,10 +154,11 @@ def variant_scene(seed: int, kind: str):
 
 @dataclass
 class Pairs:
-    noisy: object   # torch (N, H, W, 4)
-    albedo: object
-    gbuf: object
-    ref: object
+    noisy: object   # torch (L, N, H, W, 4): one input per sample count of spp_levels
+    albedo: object  # (L, N, H, W, 4)
+    gbuf: object    # (L, N, H, W, 4)
+    ref: object     # (N, H, W, 4)
+    spp_levels: tuple = (4,)
 
 
 def reference_render(ctx: Context, cam, W: int, H: int, spp_in: int, spp_ref: int, depth: int, noisy, ref,
@@ -174,26 +175,31 @@ def reference_render(ctx: Context, cam, W: int, H: int, spp_in: int, spp_ref: in
     ref.mul_(n_all / spp_ref).sub_(noisy, alpha=spp_in / spp_ref)
 
 
-def render_pairs(ctx: Context, n_variants: int, size: int = 256, spp_in: int = 4, spp_ref: int = 4096,
+def render_pairs(ctx: Context, n_variants: int, size: int = 256, spp_in=4, spp_ref: int = 4096,
                  depth: int = 8, seed0: int = 1000, log=None, nee: bool = False) -> Pairs:
     """Training pairs; with `nee`, input and reference are both rendered
-    with next-event estimation (same expectation, lower noise)."""
+    with next-event estimation (same expectation, lower noise).  `spp_in` is
+    one sample count or several: each variant then has one input per count
+    (the first s frames, 0 .. s-1) and one reference, the mean of the
+    spp_ref frames after the largest count, independent of every input."""
+    levels = tuple(int(v) for v in (spp_in if isinstance(spp_in, (list, tuple)) else (spp_in,)))
     ctx.set_nee(nee)
     try:
-        return _render_pairs(ctx, n_variants, size, spp_in, spp_ref, depth, seed0, log)
+        return _render_pairs(ctx, n_variants, size, levels, spp_ref, depth, seed0, log)
     finally:
         ctx.set_nee(False)
 
 
-def _render_pairs(ctx, n_variants, size, spp_in, spp_ref, depth, seed0, log) -> Pairs:
+def _render_pairs(ctx, n_variants, size, levels, spp_ref, depth, seed0, log) -> Pairs:
     import torch
 
     dev = torch.device("cuda:0")
-    shape = (n_variants, size, size, 4)
-    noisy = torch.empty(shape, dtype=torch.float32, device=dev)
+    L = len(levels)
+    top = levels.index(max(levels))
+    noisy = torch.empty((L, n_variants, size, size, 4), dtype=torch.float32, device=dev)
     albedo = torch.empty_like(noisy)
     gbuf = torch.empty_like(noisy)
-    ref = torch.empty_like(noisy)
+    ref = torch.empty((n_variants, size, size, 4), dtype=torch.float32, device=dev)
     s = torch.cuda.current_stream(dev).cuda_stream
     tmp_g = torch.empty((size, size, 4), dtype=torch.float32, device=dev)
     for i in range(n_variants):
@@ -201,14 +207,15 @@ def _render_pairs(ctx, n_variants, size, spp_in, spp_ref, depth, seed0, log) -> 
         loaded, sb = variant_scene(seed0 + i, kind)
         cam = cam_mod.camera_for(loaded, size, size)
         ctx.upload_scene(sb)
-        ctx.render_aux_dev(cam, size, size, spp_in, 0, depth, None, noisy[i].data_ptr(), gbuf[i].data_ptr(),
-                           albedo[i].data_ptr(), s)
-        reference_render(ctx, cam, size, size, spp_in, spp_ref, depth, noisy[i], ref[i], tmp_g, s)
+        for l, spp in enumerate(levels):
+            ctx.render_aux_dev(cam, size, size, spp, 0, depth, None, noisy[l, i].data_ptr(), gbuf[l, i].data_ptr(),
+                               albedo[l, i].data_ptr(), s)
+        reference_render(ctx, cam, size, size, levels[top], spp_ref, depth, noisy[top, i], ref[i], tmp_g, s)
         if log and (i % 16 == 0 or i == n_variants - 1):
             torch.cuda.synchronize()
             log(f"pairs {i + 1}/{n_variants}")
     torch.cuda.synchronize()
-    return Pairs(noisy, albedo, gbuf, ref)
+    return Pairs(noisy, albedo, gbuf, ref, levels)
 
 
 def train(dn: Denoiser, pairs: Pairs, steps: int, batch: int = 16, lr: float = 1e-3, seed: int = 0,
@@ -217,7 +224,7 @@ def train(dn: Denoiser, pairs: Pairs, steps: int, batch: int = 16, lr: float = 1
     loss, the rest `loss`.  The learning rate decays from lr to lr/10."""
     import torch
 
-    n = pairs.noisy.shape[0]
+    L, n = pairs.noisy.shape[0], pairs.noisy.shape[1]
     rng = np.random.default_rng(seed)
     losses = []
     dn.set_loss("log_l2" if warm_steps > 0 else loss)
@@ -225,13 +232,15 @@ def train(dn: Denoiser, pairs: Pairs, steps: int, batch: int = 16, lr: float = 1
         if it == warm_steps and warm_steps > 0:
             dn.set_loss(loss)
         idx = torch.as_tensor(rng.choice(n, size=min(batch, n), replace=False), device=pairs.noisy.device)
+        # each batch entry takes the input of a random sample count
+        lvl = torch.as_tensor(rng.integers(L, size=len(idx)), device=pairs.noisy.device)
         lr_t = lr * (0.1 ** (it / max(steps, 1)))
         # Augmentation: one of the 8 symmetries of the square tile, shared by
         # the batch (the G-buffer normals stay world-space features).
         k, flip = int(rng.integers(4)), bool(rng.integers(2))
 
         def aug(t):
-            t = t[idx]
+            t = t[lvl, idx] if t.dim() == 5 else t[idx]
             if flip:
                 t = torch.flip(t, dims=(2,))
             return torch.rot90(t, k, dims=(1, 2)).contiguous() if k else t.contiguous()
@@ -421,7 +430,7 @@ def _main(argv=None):
     sub = ap.add_subparsers(dest="cmd", required=True)
     t = sub.add_parser("train")
     t.add_argument("--variants", type=int, default=512)
-    t.add_argument("--spp-in", type=int, default=4)
+    t.add_argument("--spp-in", default="4", help="input sample count, or several separated by commas")
     t.add_argument("--spp-ref", type=int, default=4096)
     t.add_argument("--steps", type=int, default=16000)
     t.add_argument("--batch", type=int, default=16)
@@ -447,7 +456,8 @@ def _main(argv=None):
         if a.cmd == "train":
             t0 = time.time()
             out = a.out or (WEIGHTS_NEE_PATH if a.nee else WEIGHTS_PATH)
-            pairs = render_pairs(ctx, a.variants, spp_in=a.spp_in, spp_ref=a.spp_ref, log=log, nee=a.nee)
+            spp_in = [int(v) for v in str(a.spp_in).split(",")]
+            pairs = render_pairs(ctx, a.variants, spp_in=spp_in, spp_ref=a.spp_ref, log=log, nee=a.nee)
             t_pairs = time.time() - t0
             log(f"pairs rendered in {t_pairs:.1f} s")
             dn = Denoiser(ctx, seed=a.seed)
@@ -457,7 +467,7 @@ def _main(argv=None):
             torch.cuda.synchronize()
             t_train = time.time() - t1
             log(f"trained {a.steps} steps in {t_train:.1f} s")
-            save_weights(dn, out, {"variants": a.variants, "size": 256, "spp_in": a.spp_in,
+            save_weights(dn, out, {"variants": a.variants, "size": 256, "spp_in": spp_in,
                                    "spp_ref": a.spp_ref, "nee": a.nee, "steps": a.steps, "batch": a.batch,
                                    "lr": a.lr, "seed": a.seed, "loss": a.loss, "warm_steps": a.warm_steps,
                                    "loss_first": losses[0], "loss_last": losses[-1],
