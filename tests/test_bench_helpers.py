@@ -1,0 +1,68 @@
+"""bench.py's host helpers (no GPU): the counter arithmetic behind the
+roofline fields, the scene cache the counter probe loads, and the one-GPU
+contract."""
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+sys.path.insert(0, ROOT)
+
+import bench  # noqa: E402
+
+
+def test_kernel_fractions_closed_forms(monkeypatch):
+    # A copy of exactly 1 GiB that FETCH_SIZE counts at half (the part's
+    # behaviour) gives factors 2 and 1; a kernel that issues every cycle on
+    # every SIMD with all lanes active has VALU issue 1 and lane efficiency 1.
+    class Props:
+        multi_processor_count = 256
+
+    import torch
+
+    monkeypatch.setattr(torch.cuda, "get_device_properties", lambda i: Props())
+    gib_kib = (1 << 30) / 1024.0
+    cycles = 1_000_000
+    counters = {
+        bench.COPY_KERNEL: {"FETCH_SIZE": (gib_kib / 2, 6), "WRITE_SIZE": (gib_kib, 6)},
+        "k": {"FETCH_SIZE": (1000.0, 1), "WRITE_SIZE": (500.0, 1), "TCC_HIT_sum": (3.0, 1),
+              "TCC_MISS_sum": (1.0, 1), "GRBM_GUI_ACTIVE": (8.0 * cycles, 1),
+              "SQ_INSTS_VALU": (1024 * cycles / 4.0, 1), "SQ_THREAD_CYCLES_VALU": (64 * 1024 * cycles / 4.0, 1),
+              "TA_BUSY_avr": (cycles / 2.0, 1), "SQ_INSTS_VMEM_RD": (7.0, 1)},
+    }
+    f = bench.traffic_factors(counters)
+    assert f == pytest.approx((2.0, 1.0))
+    r = bench.kernel_fractions(counters, "k", avg_ms=1.0)
+    assert r["traffic_bytes"] == round(1000 * 1024 * 2 + 500 * 1024)
+    assert r["hbm_gbs"] == pytest.approx(r["traffic_bytes"] / 1e-3 / 1e9, abs=0.05)  # rounded to 0.1
+    assert r["l2_hit_rate"] == 0.75
+    assert r["valu_issue_frac"] == 1.0 and r["lane_efficiency"] == 1.0 and r["valu_lane_frac"] == 1.0
+    assert r["ta_busy_frac"] == 0.5 and r["cycles_per_xcd"] == cycles
+
+
+def test_missing_copy_calibration_gives_no_traffic():
+    r = bench.kernel_fractions({"k": {"FETCH_SIZE": (1.0, 1), "WRITE_SIZE": (1.0, 1)}}, "k", 1.0)
+    assert "traffic_bytes" not in r
+
+
+def test_probe_scene_cache_round_trip(tmp_path):
+    from mcrt_amd import configs
+
+    w = configs.c1(width=32, height=24, spp=3)
+    p = str(tmp_path / "w.npz")
+    bench.save_workload(w, p)
+    v = bench.load_workload(p)
+    for k, a in w.buffers.__dict__.items():
+        b = getattr(v.buffers, k)
+        assert (np.array_equal(a, b) if isinstance(a, np.ndarray) else a == b), k
+    assert (v.width, v.height, v.spp, v.depth, v.name) == (32, 24, 3, 8, w.name)
+    assert np.array_equal(v.camera, w.camera)
+
+
+def test_more_than_one_gpu_is_refused():
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 2 and "one GPU" in r.stderr and r.stdout == ""
