@@ -48,6 +48,8 @@ def main():
     ap.add_argument("--out", default=None)
     ap.add_argument("--cache", default=None,
                     help="directory for finished windows, so an interrupted run resumes where it stopped")
+    ap.add_argument("--only", default=None,
+                    help="comma-separated window indices: compute these into --cache and stop (no fixture)")
     a = ap.parse_args()
     w = configs.by_name(a.workload)
     out_path = a.out or os.path.join(os.path.dirname(os.path.abspath(__file__)),
@@ -58,7 +60,12 @@ def main():
     t0 = time.time()
     if a.cache:
         os.makedirs(a.cache, exist_ok=True)
+    only = None if a.only is None else {int(v) for v in a.only.split(",")}
+    if only is not None and not a.cache:
+        ap.error("--only needs --cache")
     for i, (x0, y0, cw, ch) in enumerate(wins):
+        if only is not None and i not in only:
+            continue
         cached = os.path.join(a.cache, f"{a.workload}_{a.spp}_{i}.npy") if a.cache else None
         if cached and os.path.exists(cached):
             parts.append(np.load(cached))
@@ -68,6 +75,8 @@ def main():
         if cached:
             np.save(cached, parts[-1])
         print(f"window {i + 1}/{len(wins)} at ({x0},{y0}) done, {time.time() - t0:.0f}s", flush=True)
+    if only is not None:
+        return
     if a.workload == "c1":
         image = np.concatenate(parts, axis=0).astype(np.float32)
         np.savez_compressed(out_path, image=image, spp=a.spp, first_frame=FIRST, depth=w.depth,

@@ -54,7 +54,8 @@ def test_c3_window_fixture_recomputes_exactly():
     assert crops.shape == (8, 32, 32, 3) and int(z["spp"]) == 65536
     w = configs.c3()
     sc = oracle.Scene(w.buffers)
-    x0, y0 = (int(v) for v in origins[3])
-    win = oracle.reference64_window(sc, w.camera, w.width, w.height, x0 + 30, y0 + 31, 2, 1, 65536, 1 << 24,
-                                    w.depth, threads=4)
-    assert np.array_equal(win.astype(np.float32), crops[3, 31:32, 30:32])
+    for k in (3, 6):
+        x0, y0 = (int(v) for v in origins[k])
+        win = oracle.reference64_window(sc, w.camera, w.width, w.height, x0 + 30, y0 + 31, 2, 1, 65536, 1 << 24,
+                                        w.depth, threads=4)
+        assert np.array_equal(win.astype(np.float32), crops[k, 31:32, 30:32]), k
