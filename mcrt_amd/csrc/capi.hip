@@ -1501,12 +1501,14 @@ extern "C" int mcrt_denoiser_infer_dev(mcrt_denoiser *d, uint32_t batch, uint32_
     hipStream_t s = (hipStream_t)stream;
     d->c->last_stream = s;
     // The training forward pass without its log-space output: the last
-    // layer writes the denoised image.
+    // layer writes the denoised image, and the first reads a 16-channel
+    // input (32 B per pixel instead of 64).
     PackArgs pa{(const float4 *)noisy, (const float4 *)albedo, (const float4 *)gbuf, (__bf16 *)d->act[0].p,
                 (int)batch, (int)height, (int)width};
-    HIP_TRY(d->c, launch_pack(pa, s));
+    HIP_TRY(d->c, launch_pack16(pa, s));
     for (int l = 0; l < kDenoiseLayers; ++l) {
         ConvArgs ca{};
+        ca.cin16 = l == 0;
         ca.in = (const __bf16 *)d->act[l].p;
         ca.noisy = (const float4 *)noisy;
         ca.w = (const __bf16 *)d->wf[l].p;

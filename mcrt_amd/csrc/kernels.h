@@ -170,6 +170,7 @@ struct ConvArgs {
     float4 *out_final;
     int B, H, W;
     int mode;            // 0: ReLU, BF16 out; 1: linear, BF16 out; 2: final layer
+    int cin16;           // mode 0 only: the input has 16 channels ([B][H][W][16], k_pack16)
 };
 
 struct WgradArgs {
@@ -200,6 +201,7 @@ constexpr int kDenoiseLayers = 6;
 constexpr int kLayerParams = 9 * 32 * 32 + 32;
 
 hipError_t launch_pack(const PackArgs &a, hipStream_t s);
+hipError_t launch_pack16(const PackArgs &a, hipStream_t s);  // out: [B][H][W][16]
 hipError_t launch_conv3x3(const ConvArgs &a, hipStream_t s);
 hipError_t launch_wgrad(const WgradArgs &a, uint32_t blocks, float *out_grad, hipStream_t s);
 hipError_t launch_loss(const LossArgs &a, float *loss_out, hipStream_t s);

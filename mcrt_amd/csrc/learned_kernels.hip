@@ -86,6 +86,21 @@ __global__ __launch_bounds__(256) void k_pack(PackArgs a) {
     o[3] = z;
 }
 
+// Inference input with 16 channels (32 B per pixel): the 10 network inputs
+// and 6 zeros.  The first layer's 16-channel kernel (k_conv3x3 with
+// kCinT = 16) reads it; training keeps the 32-channel layout, whose padded
+// channels the weight gradient reads.
+__global__ __launch_bounds__(256) void k_pack16(PackArgs a) {
+    const size_t n = (size_t)a.B * a.H * a.W;
+    const size_t p = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (p >= n) return;
+    bf16x8 lo, hi;
+    input_channels(a.noisy[p], a.albedo[p], a.gbuf[p], lo, hi);
+    bf16x8 *o = reinterpret_cast<bf16x8 *>(a.out + p * 16);
+    o[0] = lo;
+    o[1] = hi;
+}
+
 // ---------------------------------------------------------------------------
 // 3x3 convolution, implicit GEMM on MFMA
 // ---------------------------------------------------------------------------
@@ -108,19 +123,40 @@ __global__ __launch_bounds__(256) void k_pack(PackArgs a) {
 // (ConvArgs::out_final) instead of the log-space output, so no unpack pass
 // follows; a separate instantiation, so the training kernel carries none of
 // it.
-template <bool kFinal, bool kEnd = false>
+// kCinT = 16 (inference, first layer): the input has 16 channels (k_pack16),
+// and one MFMA's K = 32 spans two horizontal taps of 16 channels each: taps
+// (kx 0, 1) and (kx 2, zeros), 6 MFMAs per output row instead of 9, with
+// half the input bytes.  Its weights are the layer's 32-channel weights,
+// whose channels 16..31 are zero.
+template <bool kFinal, bool kEnd = false, int kCinT = 32>
 __global__ __launch_bounds__(kFinal ? 256 : 512) void k_conv3x3(ConvArgs a) {
     constexpr int kThreads = kFinal ? 256 : 512;
-    __shared__ __align__(16) bf16 s_in[kH * kH * kPS];
+    constexpr bool kIn16 = kCinT == 16;
+    constexpr int kPSin = kIn16 ? 24 : kPS;  // LDS pixel stride of the input tile
+    constexpr int kQuads = kCinT / 8;        // 16-B loads per input pixel
+    constexpr int kLdsIn = kH * kH * kPSin, kLdsOut = kT * kT * kPS;
+    __shared__ __align__(16) bf16 s_in[kLdsIn > kLdsOut ? kLdsIn : kLdsOut];
     const int tid = threadIdx.x;
     const int wave = tid >> 6, lane = tid & 63;
     const int m = lane & 15, kq = lane >> 4;
     const int rg = wave & 3;                     // row group: output rows 4*rg .. 4*rg+3
     const int half = kFinal ? 0 : (wave >> 2);   // output channels 16*half .. 16*half+15
-    bf16x8 wfrag[9];
+    // B fragments: per tap (32 channels), or for kIn16 per (ky, tap pair):
+    // lanes kq 0-1 hold tap kx = 2p channels 8kq.., lanes kq 2-3 tap 2p+1.
+    constexpr int kFrags = kIn16 ? 6 : 9;
+    bf16x8 wfrag[kFrags];
 #pragma unroll
-    for (int tap = 0; tap < 9; ++tap)
-        wfrag[tap] = *reinterpret_cast<const bf16x8 *>(a.w + (tap * kC + 16 * half + m) * kC + kq * 8);
+    for (int f = 0; f < kFrags; ++f) {
+        if (kIn16) {
+            const int ky = f >> 1, pr = f & 1;
+            const int kx = 2 * pr + (kq >> 1), ci = (kq & 1) * 8;
+            wfrag[f] = bf16x8{};
+            if (kx < 3)
+                wfrag[f] = *reinterpret_cast<const bf16x8 *>(a.w + ((ky * 3 + kx) * kC + 16 * half + m) * kC + ci);
+        } else {
+            wfrag[f] = *reinterpret_cast<const bf16x8 *>(a.w + (f * kC + 16 * half + m) * kC + kq * 8);
+        }
+    }
     const float bias = a.bias ? a.bias[16 * half + m] : 0.0f;
 
     const int tiles_x = (a.W + kT - 1) / kT, tiles_y = (a.H + kT - 1) / kT;
@@ -128,7 +164,7 @@ __global__ __launch_bounds__(kFinal ? 256 : 512) void k_conv3x3(ConvArgs a) {
     // Software pipeline: each thread's share of a tile's input (with halo,
     // zero outside the image) is loaded into registers one tile ahead, so
     // the loads are in flight while the current tile computes.
-    constexpr int kLoads = (kH * kH * 4 + kThreads - 1) / kThreads;
+    constexpr int kLoads = (kH * kH * kQuads + kThreads - 1) / kThreads;
     uint4 pre[kLoads], pmask[kLoads];
     auto fetch = [&](int t) {
         const int b = t / per_image, rem = t - b * per_image;
@@ -136,16 +172,16 @@ __global__ __launch_bounds__(kFinal ? 256 : 512) void k_conv3x3(ConvArgs a) {
 #pragma unroll
         for (int k = 0; k < kLoads; ++k) {
             const int i = tid + k * kThreads;
-            const int pix = i >> 2, q = i & 3;
+            const int pix = i / kQuads, q = i % kQuads;
             const int hy = pix / kH, hx = pix - hy * kH;
             const int gy = y0 + hy - 1, gx = x0 + hx - 1;
             pre[k] = make_uint4(0, 0, 0, 0);
             pmask[k] = make_uint4(0, 0, 0, 0);
-            if (i < kH * kH * 4 && gy >= 0 && gy < a.H && gx >= 0 && gx < a.W) {
+            if (i < kH * kH * kQuads && gy >= 0 && gy < a.H && gx >= 0 && gx < a.W) {
                 const size_t px = ((size_t)b * a.H + gy) * a.W + gx;
-                const size_t off = px * kC + q * 8;
+                const size_t off = px * kCinT + q * 8;
                 pre[k] = *reinterpret_cast<const uint4 *>(a.in + off);
-                if (a.mask) pmask[k] = *reinterpret_cast<const uint4 *>(a.mask + off);
+                if (!kIn16 && a.mask) pmask[k] = *reinterpret_cast<const uint4 *>(a.mask + off);
             }
         }
     };
@@ -157,9 +193,10 @@ __global__ __launch_bounds__(kFinal ? 256 : 512) void k_conv3x3(ConvArgs a) {
 #pragma unroll
         for (int k = 0; k < kLoads; ++k) {
             const int i = tid + k * kThreads;
-            if (i < kH * kH * 4) {
-                const int pix = i >> 2, q = i & 3;
-                *reinterpret_cast<uint4 *>(&s_in[pix * kPS + q * 8]) = a.mask ? apply_mask(pre[k], pmask[k]) : pre[k];
+            if (i < kH * kH * kQuads) {
+                const int pix = i / kQuads, q = i % kQuads;
+                *reinterpret_cast<uint4 *>(&s_in[pix * kPSin + q * 8]) =
+                    (!kIn16 && a.mask) ? apply_mask(pre[k], pmask[k]) : pre[k];
             }
         }
         __syncthreads();
@@ -169,15 +206,32 @@ __global__ __launch_bounds__(kFinal ? 256 : 512) void k_conv3x3(ConvArgs a) {
         for (int r = 0; r < 4; ++r) acc[r] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
         for (int ir = 0; ir < 6; ++ir) {
+            if constexpr (kIn16) {
 #pragma unroll
-            for (int kx = 0; kx < 3; ++kx) {
-                const bf16x8 av =
-                    *reinterpret_cast<const bf16x8 *>(&s_in[((rg * 4 + ir) * kH + (m + kx)) * kPS + kq * 8]);
+                for (int pr = 0; pr < 2; ++pr) {
+                    const int kx = 2 * pr + (kq >> 1);
+                    bf16x8 av = bf16x8{};
+                    if (kx < 3)
+                        av = *reinterpret_cast<const bf16x8 *>(
+                            &s_in[((rg * 4 + ir) * kH + (m + kx)) * kPSin + (kq & 1) * 8]);
 #pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    const int ky = ir - r;
-                    if (ky < 0 || ky > 2) continue;
-                    acc[r] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, wfrag[ky * 3 + kx], acc[r], 0, 0, 0);
+                    for (int r = 0; r < 4; ++r) {
+                        const int ky = ir - r;
+                        if (ky < 0 || ky > 2) continue;
+                        acc[r] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, wfrag[ky * 2 + pr], acc[r], 0, 0, 0);
+                    }
+                }
+            } else {
+#pragma unroll
+                for (int kx = 0; kx < 3; ++kx) {
+                    const bf16x8 av =
+                        *reinterpret_cast<const bf16x8 *>(&s_in[((rg * 4 + ir) * kH + (m + kx)) * kPS + kq * 8]);
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const int ky = ir - r;
+                        if (ky < 0 || ky > 2) continue;
+                        acc[r] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, wfrag[ky * 3 + kx], acc[r], 0, 0, 0);
+                    }
                 }
             }
         }
@@ -453,6 +507,11 @@ hipError_t launch_pack(const PackArgs &a, hipStream_t s) {
     return hipGetLastError();
 }
 
+hipError_t launch_pack16(const PackArgs &a, hipStream_t s) {
+    hipLaunchKernelGGL(k_pack16, dim3(blocks_for((size_t)a.B * a.H * a.W)), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
 // Grid-stride launch sized to the blocks that are resident at once.
 template <bool kFinal>
 uint32_t conv_grid(size_t tiles) {
@@ -477,6 +536,8 @@ hipError_t launch_conv3x3(const ConvArgs &a, hipStream_t s) {
         hipLaunchKernelGGL((k_conv3x3<true, true>), dim3(conv_grid<true>(tiles)), dim3(256), 0, s, a);
     else if (a.mode == 2)
         hipLaunchKernelGGL((k_conv3x3<true, false>), dim3(conv_grid<true>(tiles)), dim3(256), 0, s, a);
+    else if (a.cin16)
+        hipLaunchKernelGGL((k_conv3x3<false, false, 16>), dim3(conv_grid<false>(tiles)), dim3(512), 0, s, a);
     else
         hipLaunchKernelGGL((k_conv3x3<false, false>), dim3(conv_grid<false>(tiles)), dim3(512), 0, s, a);
     return hipGetLastError();
