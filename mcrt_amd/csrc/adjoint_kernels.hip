@@ -113,18 +113,18 @@ __device__ __forceinline__ V3 diff_path_radiance(const DiffArgs &a, V3 o, V3 d, 
     return L;
 }
 
-// The kernels are compiled for more waves per SIMD than the compiler's own
-// choice (primal 5 waves at 89 VGPRs, adjoint 4 at 122), spilling part of
+// The kernels are compiled for 8 waves per SIMD, more than the compiler's
+// own choice (primal 5 waves at 89 VGPRs, adjoint 4 at 122), spilling part of
 // the 32 gradient accumulators and the traversal state: more waves hide the
 // latency of dependent node fetches better than the spills cost (the 512^2
 // inverse step went 26.8 -> 21.5 ms, each step up in waves faster than the
-// last).  Primal and tape adjoint run 8 waves (64 VGPRs; 48 and 160 B of
-// spill per lane).  The replay adjoint also carries the 24 parameter
-// derivatives: at 7 waves (72 VGPRs, 228 B spill) it takes 12.7 ms per
-// 512^2 x 32 spp call against 13.5 at 8, 15.0 at 6 and 14.2 at 5.
+// last).  The replay adjoint also carries the 24 parameter derivatives; with
+// the while-while traversal of trace_one it takes 10.1 ms per 512^2 x 32 spp
+// call at 8 waves, 10.4 at 7 and 10.2 at 6 (the primal 7.8 ms at 8, 8.2 at 7,
+// 8.8 at 6).
 // kMode: 0 primal, 1 replay adjoint, 2 tape adjoint.
 template <int kBlock, bool kSpheres, int kMode>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kMode == 1 ? 7 : 8, kMode == 1 ? 7 : 8))) void k_diff(DiffArgs a) {
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_diff(DiffArgs a) {
     constexpr bool kAdjoint = kMode != 0;
     constexpr int kWaves = kBlock / 64;
     __shared__ uint32_t lds[Stack::kShort * Stack::kWords * kBlock];

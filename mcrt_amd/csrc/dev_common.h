@@ -386,43 +386,62 @@ __device__ __forceinline__ void sphere_leaf(const DevScene &sc, uint32_t i, V3 o
     }
 }
 
-// Closest hit of one ray by one thread (world_hit semantics and visit rule of
+// Closest hit of one ray per lane (world_hit semantics and visit rule of
 // trace_persistent), for megakernels that keep a whole path in one thread.
+// Called by every active lane of a wave together (it votes with ballots);
+// its loop has trace_persistent's while-while order without the refill:
+// lanes walk internal nodes, postponing the first leaf they reach, until
+// every lane holds a leaf or is done, then the leaves are intersected
+// together, so lanes at leaves and lanes at nodes do not take turns.
 template <bool kSpheres>
 __device__ __forceinline__ Best trace_one(const DevScene &sc, V3 o, V3 d, float t_min, float t_max,
                                           const Stack &st) {
     const V3 inv = v3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
-    Best sb;
-    sb.t = t_max;
-    sb.id = sb.prim = kNone;
-    sb.u = sb.v = 0.0f;
-    uint32_t sp = 0;
-    if (kSpheres && sc.sroot != kNone) {
-        uint32_t cur = enter_root(sc, true, o, inv, t_min, t_max);
-        while (cur != kNone) {
-            if (!(cur & kLeafBit)) {
-                node_step(sc.snodes, o, inv, t_min, t_max, sb.t, cur, sp, st);
-            } else {
-                sphere_leaf(sc, cur & ~kLeafBit, o, d, t_min, t_max, sb);
-                cur = st.pop(sp, sb.t * kCullSlack);
+    Best best, sbest;
+    best.t = t_max;
+    best.id = best.prim = kNone;
+    best.u = best.v = 0.0f;
+    sbest = best;
+    bool sphere_phase = kSpheres && sc.sroot != kNone;
+    float t_cur = t_max;
+    uint32_t cur = enter_root(sc, sphere_phase, o, inv, t_min, t_cur);
+    uint32_t sp = 0, leaf = kNone;
+    bool done = false;
+    for (;;) {
+        for (;;) {
+            if (!done && leaf == kNone && cur != kNone && (cur & kLeafBit)) {
+                leaf = cur;
+                cur = st.pop(sp, best.t * kCullSlack);
+            }
+            const bool in = !done && cur != kNone && !(cur & kLeafBit);
+            if (__ballot(in && leaf == kNone) == 0) break;
+            if (in) {
+                const float4 *nodes = (kSpheres && sphere_phase) ? sc.snodes : sc.mnodes;
+                node_step(nodes, o, inv, t_min, t_cur, best.t, cur, sp, st);
             }
         }
-    }
-    const float t_mesh = sb.prim != kNone ? sb.t : t_max;
-    Best mb;
-    mb.t = t_mesh;
-    mb.id = mb.prim = kNone;
-    mb.u = mb.v = 0.0f;
-    uint32_t cur = enter_root(sc, false, o, inv, t_min, t_mesh);
-    while (cur != kNone) {
-        if (!(cur & kLeafBit)) {
-            node_step(sc.mnodes, o, inv, t_min, t_mesh, mb.t, cur, sp, st);
-        } else {
-            tri_leaf(sc, cur & ~kLeafBit, o, d, t_min, t_mesh, mb);
-            cur = st.pop(sp, mb.t * kCullSlack);
+        if (!done && leaf != kNone) {
+            if (kSpheres && sphere_phase)
+                sphere_leaf(sc, leaf & ~kLeafBit, o, d, t_min, t_cur, best);
+            else
+                tri_leaf(sc, leaf & ~kLeafBit, o, d, t_min, t_cur, best);
+            leaf = kNone;
         }
+        if (!done && cur == kNone && leaf == kNone) {
+            if (kSpheres && sphere_phase) {  // spheres -> mesh (world_hit, :471-479)
+                sbest = best;
+                sphere_phase = false;
+                t_cur = sbest.prim != kNone ? sbest.t : t_max;
+                best.prim = kNone;
+                best.id = kNone;
+                best.t = t_cur;
+                cur = enter_root(sc, false, o, inv, t_min, t_cur);
+            }
+            if (!sphere_phase && cur == kNone) done = true;
+        }
+        if (__ballot(!done) == 0) break;
     }
-    return mb.prim != kNone ? mb : sb;
+    return (!kSpheres || best.prim != kNone) ? best : sbest;
 }
 
 // IO policy: count(), work(), load(slot, o, d, t_min, t_max), store(slot, best),
