@@ -341,6 +341,25 @@ int convert_tree(mcrt_ctx *c, const char *what, const GpuBvhNode *nodes, uint32_
     return MCRT_OK;
 }
 
+// The child pairs of a converted tree as quantized 32-B pairs
+// (node_quant.h); unused slots stay zero.  False where a box cannot be
+// encoded.
+bool quantize_tree(const std::vector<float4> &nodes, std::vector<uint32_t> &out) {
+    const size_t n = nodes.size() / 4;
+    out.assign(std::max<size_t>(n, 1) * kQuantWords, 0u);
+    for (size_t i = 0; i < n; ++i) {
+        const float4 *q = &nodes[4 * i];
+        uint32_t r0, r1;
+        std::memcpy(&r0, &q[3].x, 4);
+        std::memcpy(&r1, &q[3].y, 4);
+        if (r0 == 0u && r1 == 0u) continue;  // an unused slot (child 0 is never the root)
+        const float lo0[3] = {q[0].x, q[0].z, q[2].x}, hi0[3] = {q[0].y, q[0].w, q[2].y};
+        const float lo1[3] = {q[1].x, q[1].z, q[2].z}, hi1[3] = {q[1].y, q[1].w, q[2].w};
+        if (!qnode_encode(lo0, hi0, lo1, hi1, r0, r1, &out[kQuantWords * i])) return false;
+    }
+    return true;
+}
+
 }  // namespace
 
 extern "C" int mcrt_validate_bvh(const GpuBvhNode *nodes, uint32_t n_nodes, uint32_t n_prims) {
@@ -612,10 +631,27 @@ extern "C" int mcrt_upload_scene(mcrt_ctx *c, const GpuSphere *spheres, uint32_t
         }
     }
 
-    if ((rc = upload(c, c->mnodes, mt.nodes.data(), mt.nodes.size() * sizeof(float4)))) return rc;
+    // Scenes that spill the L2 traverse quantized 32-B child pairs with the
+    // culling stack (voxel terrain 1186 -> 1237 Msamples/s: half the node
+    // bytes per step, and the address unit is the busiest unit of the
+    // traversal); cache-resident scenes keep the 64-B pairs, which decode for
+    // free (icosphere Cornell: quantized pairs cost 4% of the render and 9% of
+    // the 16M-ray closest-hit launch).  A tree with a box the grid cannot
+    // encode keeps the 64-B pairs and the plain stack.
+    std::vector<uint32_t> qm, qs;
+    const bool quant = !fits_l2 && quantize_tree(mt.nodes, qm) && quantize_tree(st.nodes, qs);
+    if (quant) {
+        if ((rc = upload(c, c->mnodes, qm.data(), qm.size() * sizeof(uint32_t)))) return rc;
+    } else {
+        if ((rc = upload(c, c->mnodes, mt.nodes.data(), mt.nodes.size() * sizeof(float4)))) return rc;
+    }
     if ((rc = upload(c, c->mtris, dt.data(), dt.size() * sizeof(float4)))) return rc;
     if ((rc = upload(c, c->tshade, dsh.data(), dsh.size() * sizeof(float4)))) return rc;
-    if ((rc = upload(c, c->snodes, st.nodes.data(), st.nodes.size() * sizeof(float4)))) return rc;
+    if (quant) {
+        if ((rc = upload(c, c->snodes, qs.data(), qs.size() * sizeof(uint32_t)))) return rc;
+    } else {
+        if ((rc = upload(c, c->snodes, st.nodes.data(), st.nodes.size() * sizeof(float4)))) return rc;
+    }
     if ((rc = upload(c, c->sph, ds.data(), ds.size() * sizeof(float4)))) return rc;
     if ((rc = upload(c, c->mats, materials->materials, sizeof(materials->materials)))) return rc;
     if ((rc = upload(c, c->env, denv.data(), denv.size() * sizeof(float4)))) return rc;
@@ -662,7 +698,8 @@ extern "C" int mcrt_upload_scene(mcrt_ctx *c, const GpuSphere *spheres, uint32_t
     // Culling pops skip node fetches that miss the L2 on such scenes (voxel
     // terrain: +2.8%); on a cache-resident scene the bookkeeping and the
     // shorter LDS stack cost more than they save (icosphere Cornell: -3.5%).
-    d.stack_cull = fits_l2 ? 0u : 1u;
+    // The culling-stack kernels read the quantized pairs.
+    d.stack_cull = quant ? 1u : 0u;
     if (d.stack_depth > 96)
         return fail(c, MCRT_ERR_LIMIT, "BVH depth %u exceeds the traversal stack limit of 96",
                     d.stack_depth);

@@ -14,6 +14,7 @@
 #include <stdint.h>
 
 #include "../../include/mcrt_abi.h"
+#include "node_quant.h"
 
 namespace mcrt {
 
@@ -27,6 +28,9 @@ namespace mcrt {
 //   q1 = (c1.lo.x, c1.hi.x, c1.lo.y, c1.hi.y)
 //   q2 = (c0.lo.z, c0.hi.z, c1.lo.z, c1.hi.z)
 //   q3 = (ref0, ref1, -, -) as uint bits
+// Scenes whose traversal set spills the L2 (DevScene::stack_cull) store the
+// same pairs quantized to 32 B instead (node_quant.h): half the bytes per
+// node step, boxes that contain the reference boxes.
 // A child reference is a device node index, or kLeafBit | first primitive
 // index in leaf order.  A leaf's primitives run until the one whose
 // kPrimLast flag is set.
@@ -67,7 +71,8 @@ struct DevScene {
     uint32_t env_w, env_h;
     uint32_t stack_depth;     // traversal stack entries needed per lane
     uint32_t refill;          // idle lanes (1..64) at which a traversal wave loads new rays
-    uint32_t stack_cull;      // traversal kernels with the culling stack (TStack<true>)
+    uint32_t stack_cull;      // traversal kernels with the culling stack (TStack<true>), and
+                              // quantized 32-B child pairs in mnodes / snodes (node_quant.h)
     // Emitters for next-event estimation: emissive triangles and spheres,
     // chosen with probability proportional to area x luminance.
     //   emit[2e] = (prob, kind | leaf-order index bits, area, -),
@@ -299,15 +304,35 @@ __device__ __forceinline__ uint32_t enter_root(const DevScene &sc, bool sphere, 
 }
 
 // One internal-node step: slab-test both children, descend near-first.
-template <class S>
+// kQuant: the node is a quantized 32-B pair (two 16-B loads instead of four);
+// its decoded boxes contain the reference boxes, and the slab test on them is
+// the reference's.
+template <bool kQuant, class S>
 __device__ __forceinline__ void node_step(const float4 *nodes, V3 o, V3 inv, float t_min,
                                           float t_max, float best_t, uint32_t &cur, uint32_t &sp,
                                           const S &st) {
-    const float4 *n = nodes + 4u * cur;
-    float4 q0 = n[0], q1 = n[1], q2 = n[2], q3 = n[3];
     float te0, tx0, te1, tx1;
-    slab2(o, inv, q0.x, q0.y, q0.z, q0.w, q2.x, q2.y, te0, tx0);
-    slab2(o, inv, q1.x, q1.y, q1.z, q1.w, q2.z, q2.w, te1, tx1);
+    uint32_t c0, c1;
+    if constexpr (kQuant) {
+        const uint4 *n = reinterpret_cast<const uint4 *>(nodes) + 2u * cur;
+        const uint4 wa = n[0], wb = n[1];
+        const float sx = qnode_step(wa.x), ox = qnode_origin(wa.x);
+        const float sy = qnode_step(wa.y), oy = qnode_origin(wa.y);
+        const float sz = qnode_step(wa.z), oz = qnode_origin(wa.z);
+        slab2(o, inv, qnode_plane(wa.w, 0, sx, ox), qnode_plane(wa.w, 1, sx, ox), qnode_plane(wa.w, 2, sy, oy),
+              qnode_plane(wa.w, 3, sy, oy), qnode_plane(wb.x, 0, sz, oz), qnode_plane(wb.x, 1, sz, oz), te0, tx0);
+        slab2(o, inv, qnode_plane(wb.x, 2, sx, ox), qnode_plane(wb.x, 3, sx, ox), qnode_plane(wb.y, 0, sy, oy),
+              qnode_plane(wb.y, 1, sy, oy), qnode_plane(wb.y, 2, sz, oz), qnode_plane(wb.y, 3, sz, oz), te1, tx1);
+        c0 = wb.z;
+        c1 = wb.w;
+    } else {
+        const float4 *n = nodes + 4u * cur;
+        float4 q0 = n[0], q1 = n[1], q2 = n[2], q3 = n[3];
+        slab2(o, inv, q0.x, q0.y, q0.z, q0.w, q2.x, q2.y, te0, tx0);
+        slab2(o, inv, q1.x, q1.y, q1.z, q1.w, q2.z, q2.w, te1, tx1);
+        c0 = __float_as_uint(q3.x);
+        c1 = __float_as_uint(q3.y);
+    }
     const float lim = best_t * kCullSlack;
     // te < t_max (the reference test) and !(te > lim) (the cull) as one
     // compare: te < min(t_max, next float above lim) holds exactly when both
@@ -315,7 +340,6 @@ __device__ __forceinline__ void node_step(const float4 *nodes, V3 o, V3 inv, flo
     const float bound = fminf(t_max, __uint_as_float(__float_as_uint(lim) + 1u));
     bool h0 = tx0 > fmaxf(te0, t_min) && te0 < bound;
     bool h1 = tx1 > fmaxf(te1, t_min) && te1 < bound;
-    uint32_t c0 = __float_as_uint(q3.x), c1 = __float_as_uint(q3.y);
     if (h0 && h1) {
         uint32_t nearc = c0, farc = c1;
         if (te1 < te0) {
@@ -417,7 +441,10 @@ __device__ __forceinline__ Best trace_one(const DevScene &sc, V3 o, V3 d, float 
             if (__ballot(in && leaf == kNone) == 0) break;
             if (in) {
                 const float4 *nodes = (kSpheres && sphere_phase) ? sc.snodes : sc.mnodes;
-                node_step(nodes, o, inv, t_min, t_cur, best.t, cur, sp, st);
+                if (sc.stack_cull)
+                    node_step<true>(nodes, o, inv, t_min, t_cur, best.t, cur, sp, st);
+                else
+                    node_step<false>(nodes, o, inv, t_min, t_cur, best.t, cur, sp, st);
             }
         }
         if (!done && leaf != kNone) {
@@ -534,7 +561,7 @@ __device__ __forceinline__ void trace_persistent(const DevScene &sc, const IO &i
             if (__ballot(in && leaf == kNone) == 0) break;
             if (in) {
                 const float4 *nodes = (kSpheres && sphere_phase) ? sc.snodes : sc.mnodes;
-                node_step(nodes, o, inv, t_min, t_cur, best.t, cur, sp, st);
+                node_step<kCull>(nodes, o, inv, t_min, t_cur, best.t, cur, sp, st);
             }
         }
         // ---- leaves ----------------------------------------------------------
