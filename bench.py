@@ -49,12 +49,12 @@ sys.path.insert(0, ROOT)
 METRIC = "Msamples/s at 1080p; time to relMSE≤1e-2, learned vs bilateral denoise"
 HBM_PEAK_GBS = 8000.0          # MI355X HBM3E, published
 MFMA_BF16_PEAK_TFLOPS = 2500.0  # dense BF16, published
-FP32_VALU_PEAK_TFLOPS = 78.6    # non-packed FP32 FMA: 256 CUs x 128 FLOP/clk x 2.4 GHz
 N_XCD = 8
 COPY_PROBE_BYTES = 1 << 30
 EXTEND_C3 = "k_extend<256, false, true>"   # culling stack (the terrain spills the L2)
 EXTEND_C2 = "k_extend<256, false, false>"  # plain stack (cache-resident scene)
 COPY_KERNEL = "k_copy_probe"
+VALU_KERNEL = "k_valu_probe"
 PMC_PASSES = [
     ["FETCH_SIZE"],
     ["WRITE_SIZE", "TCC_HIT_sum", "TCC_MISS_sum"],
@@ -134,7 +134,9 @@ def probe(cache_dir):
 
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
-    measure_peaks(0, COPY_PROBE_BYTES)  # its copy dispatches calibrate FETCH_SIZE / WRITE_SIZE
+    # its copy dispatches calibrate FETCH_SIZE / WRITE_SIZE, its FP32 fma
+    # dispatches the cycles one VALU instruction holds a SIMD
+    measure_peaks(0, COPY_PROBE_BYTES)
     with Context(0) as ctx:
         for name in ("c3", "c2"):
             w = load_workload(os.path.join(cache_dir, f"{name}.npz"))
@@ -306,7 +308,7 @@ def counter_passes(w, keep_dir=None):
                         means.setdefault(kern, {})[c] = (float(np.mean(vals)), len(vals))
         by_short = {}
         for kern, cs in means.items():
-            for short in (EXTEND_C3, EXTEND_C2, COPY_KERNEL, "k_shade"):
+            for short in (EXTEND_C3, EXTEND_C2, COPY_KERNEL, VALU_KERNEL, "k_shade"):
                 if short in kern:
                     by_short.setdefault(short, {}).update(cs)
         return by_short
@@ -330,6 +332,19 @@ def traffic_factors(counters):
     return (COPY_PROBE_BYTES / (cp["FETCH_SIZE"][0] * 1024.0), COPY_PROBE_BYTES / (cp["WRITE_SIZE"][0] * 1024.0))
 
 
+def valu_cycles_per_inst(counters):
+    """Cycles one wave64 VALU instruction holds a SIMD, measured on the probe
+    of independent FP32 fma chains at 8 waves per SIMD (issue-bound by
+    construction): SIMDs x cycles / instructions.  None without its counters."""
+    import torch
+
+    v = counters.get(VALU_KERNEL, {})
+    if "SQ_INSTS_VALU" not in v or "GRBM_GUI_ACTIVE" not in v:
+        return None
+    simds = torch.cuda.get_device_properties(0).multi_processor_count * 4
+    return simds * (v["GRBM_GUI_ACTIVE"][0] / N_XCD) / max(v["SQ_INSTS_VALU"][0], 1.0)
+
+
 def kernel_fractions(counters, kern, avg_ms):
     """HBM traffic and VALU figures of one kernel from the counter means."""
     c = counters.get(kern, {})
@@ -351,14 +366,19 @@ def kernel_fractions(counters, kern, avg_ms):
         simds = torch.cuda.get_device_properties(0).multi_processor_count * 4
         cycles = c["GRBM_GUI_ACTIVE"][0] / N_XCD  # the counter sums the 8 XCDs
         insts = c["SQ_INSTS_VALU"][0]
-        # one wave64 VALU instruction occupies a 16-lane SIMD for 4 cycles
-        res["valu_issue_frac"] = round(insts * 4.0 / (simds * cycles), 4)
+        # VALU instruction rate against the rate of the FP32 fma probe in the
+        # same counter pass (full waves, issue-bound).  Not a busy fraction:
+        # it exceeds 1 where instructions run with idle lane groups or an
+        # empty exec mask, which issue in fewer cycles than a full-wave fma.
+        cpi = valu_cycles_per_inst(counters)
+        if cpi:
+            res["valu_cycles_per_inst_probe"] = round(cpi, 3)
+            res["valu_inst_rate_vs_fma_probe"] = round(insts * cpi / (simds * cycles), 4)
         res["valu_insts"] = round(insts)
         if "SQ_THREAD_CYCLES_VALU" in c:
             lanes = c["SQ_THREAD_CYCLES_VALU"][0] / max(insts, 1.0)
             res["active_lanes_per_valu"] = round(lanes, 2)
             res["lane_efficiency"] = round(lanes / 64.0, 4)
-            res["valu_lane_frac"] = round(res["valu_issue_frac"] * lanes / 64.0, 4)
         if "TA_BUSY_avr" in c:
             res["ta_busy_frac"] = round(c["TA_BUSY_avr"][0] / cycles, 4)
         if "SQ_INSTS_VMEM_RD" in c:
@@ -370,15 +390,17 @@ def kernel_fractions(counters, kern, avg_ms):
 def measured_peaks():
     from mcrt_amd._lib import measure_peaks
 
-    log("peaks: copy and MFMA probes")
-    gbs, tflops, tflops16 = measure_peaks(0, COPY_PROBE_BYTES)
+    log("peaks: copy, MFMA and VALU probes")
+    gbs, tflops, tflops16, valu = measure_peaks(0, COPY_PROBE_BYTES)
     return {"hbm_copy_gbs": round(gbs, 1), "hbm_copy_frac_of_published": round(gbs / HBM_PEAK_GBS, 4),
             "mfma_bf16_tflops": round(tflops, 1),
             "mfma_bf16_frac_of_published": round(tflops / MFMA_BF16_PEAK_TFLOPS, 4),
             "mfma_bf16_16x16x32_tflops": round(tflops16, 1),
+            "valu_fp32_tflops": round(valu, 1),
             "method": "mcrt_measure_peaks: one-float4-per-thread copy of 1 GiB (read + write counted), "
                       "register-operand MFMA loops on every SIMD (v_mfma_f32_32x32x16_bf16; "
-                      "v_mfma_f32_16x16x32_bf16, the convolutions' shape); HIP events, best of several runs"}
+                      "v_mfma_f32_16x16x32_bf16, the convolutions' shape), independent non-packed FP32 fma "
+                      "chains at 8 waves per SIMD; HIP events, best of several runs"}
 
 
 def roofline(ctx, w, step, segments, counters, peaks):

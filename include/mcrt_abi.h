@@ -408,12 +408,13 @@ int mcrt_render_counters(mcrt_ctx *ctx, uint64_t *segments, uint64_t *samples);
 
 /* Peak probes for roofline fractions, no counterpart in the reference: a
  * one-float4-per-thread copy between two buffers of copy_bytes each (GB/s of
- * bytes read + written) and loops of register-operand MFMAs on every SIMD:
+ * bytes read + written), loops of register-operand MFMAs on every SIMD:
  * v_mfma_f32_32x32x16_bf16 (the dense BF16 peak) and v_mfma_f32_16x16x32_bf16
- * (TFLOP/s), timed with HIP events, best of several runs.  Allocates
+ * (TFLOP/s), and independent chains of non-packed FP32 fma at 8 waves per
+ * SIMD (TFLOP/s), timed with HIP events, best of several runs.  Allocates
  * 2 x copy_bytes on `device` for the call; blocks. */
 int mcrt_measure_peaks(int device, uint64_t copy_bytes, double *copy_gbs, double *mfma_bf16_tflops,
-                       double *mfma_bf16_16x16x32_tflops);
+                       double *mfma_bf16_16x16x32_tflops, double *valu_fp32_tflops);
 
 #ifdef __cplusplus
 }

@@ -84,7 +84,7 @@ def load() -> C.CDLL:
         "mcrt_reset_stats": (C.c_int, [_vp]),
         "mcrt_render_counters": (C.c_int, [_vp, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
         "mcrt_measure_peaks": (C.c_int, [C.c_int, C.c_uint64, C.POINTER(C.c_double), C.POINTER(C.c_double),
-                                         C.POINTER(C.c_double)]),
+                                         C.POINTER(C.c_double), C.POINTER(C.c_double)]),
         "mcrt_diff_render": (C.c_int, [_vp, _vp, _u32, _u32, _u32, _u32, _u32, _vp, _u32, _vp, _vp]),
         "mcrt_diff_render_dev": (C.c_int, [_vp, _vp, _u32, _u32, _u32, _u32, _u32, _vp, _u32, _vp, _vp, _vp]),
         "mcrt_diff_adjoint": (C.c_int, [_vp, _vp, _u32, _u32, _u32, _u32, _u32, _vp, _u32, _vp, _vp, _vp]),
@@ -130,14 +130,14 @@ def exported_symbols() -> list:
 
 
 def measure_peaks(device: int = 0, copy_bytes: int = 1 << 30):
-    """(copy GB/s, BF16 MFMA TFLOP/s with 32x32x16, with 16x16x32) measured
-    on `device`."""
+    """(copy GB/s, BF16 MFMA TFLOP/s with 32x32x16, with 16x16x32, non-packed
+    FP32 VALU TFLOP/s) measured on `device`."""
     lib = load()
-    g, t, t16 = C.c_double(0.0), C.c_double(0.0), C.c_double(0.0)
-    rc = lib.mcrt_measure_peaks(device, copy_bytes, C.byref(g), C.byref(t), C.byref(t16))
+    g, t, t16, v = C.c_double(0.0), C.c_double(0.0), C.c_double(0.0), C.c_double(0.0)
+    rc = lib.mcrt_measure_peaks(device, copy_bytes, C.byref(g), C.byref(t), C.byref(t16), C.byref(v))
     if rc != MCRT_OK:
         raise McrtError(rc, "mcrt_measure_peaks failed")
-    return g.value, t.value, t16.value
+    return g.value, t.value, t16.value, v.value
 
 
 def diff_tape_bytes(width: int, height: int, spp: int, max_depth: int) -> int:

@@ -12,7 +12,12 @@
 //    reaches the dense BF16 peak, and v_mfma_f32_16x16x32_bf16 (16,384 FLOP),
 //    the shape of the learned denoiser's convolutions, which issues at
 //    about 60% of it.
-// Both are timed with HIP events on the null stream.
+//  - k_valu_probe: every lane runs independent chains of FP32 fma (not
+//    packed: the library is built without packed FP32 instructions), 8 waves
+//    per SIMD: the VALU rate of the non-packed FP32 instructions the path
+//    tracer issues.  The benchmark's counter pass also runs it to convert
+//    SQ_INSTS_VALU of a kernel into a fraction of VALU issue.
+// All are timed with HIP events on the null stream.
 #include <hip/hip_runtime.h>
 
 #include "../../include/mcrt_abi.h"
@@ -70,6 +75,24 @@ __global__ __launch_bounds__(256) void k_mfma_probe(uint32_t iters, float seed, 
     if (s == 12345.678f) out[blockIdx.x * 256u + threadIdx.x] = s;
 }
 
+constexpr int kValuChains = 8;
+
+__global__ __launch_bounds__(256) void k_valu_probe(uint32_t iters, float seed, float *out) {
+    float acc[kValuChains];
+#pragma unroll
+    for (int c = 0; c < kValuChains; ++c) acc[c] = seed * (float)(threadIdx.x + c);
+    const float m = 1.0f - seed, a = seed;
+    for (uint32_t it = 0; it < iters; ++it) {
+#pragma unroll
+        for (int c = 0; c < kValuChains; ++c) acc[c] = __builtin_fmaf(acc[c], m, a);
+    }
+    float s = 0.0f;
+#pragma unroll
+    for (int c = 0; c < kValuChains; ++c) s += acc[c];
+    // Keeps the chains live; the host never reads a value it depends on.
+    if (s == 12345.678f) out[blockIdx.x * 256u + threadIdx.x] = s;
+}
+
 }  // namespace mcrt
 
 using namespace mcrt;
@@ -87,8 +110,9 @@ struct EventPair {
 }  // namespace
 
 extern "C" int mcrt_measure_peaks(int device, uint64_t copy_bytes, double *copy_gbs, double *mfma_bf16_tflops,
-                                  double *mfma_bf16_16x16x32_tflops) {
-    if (!copy_gbs || !mfma_bf16_tflops || !mfma_bf16_16x16x32_tflops || copy_bytes < (1u << 20))
+                                  double *mfma_bf16_16x16x32_tflops, double *valu_fp32_tflops) {
+    if (!copy_gbs || !mfma_bf16_tflops || !mfma_bf16_16x16x32_tflops || !valu_fp32_tflops ||
+        copy_bytes < (1u << 20))
         return MCRT_ERR_BAD_ARG;
     if (hipSetDevice(device) != hipSuccess) return MCRT_ERR_NO_DEVICE;
     hipDeviceProp_t prop;
@@ -100,6 +124,7 @@ extern "C" int mcrt_measure_peaks(int device, uint64_t copy_bytes, double *copy_
     EventPair ev;
     const uint32_t cu = (uint32_t)prop.multiProcessorCount;
     const uint32_t mfma_blocks = cu * 4u;  // 4 waves per SIMD
+    const uint32_t valu_blocks = cu * 8u;  // 8 waves per SIMD
     auto best_ms = [&](auto launch, int runs) -> float {
         float best = 1e30f;
         for (int r = 0; r < runs + 1 && rc == MCRT_OK; ++r) {  // the first run warms up
@@ -116,7 +141,7 @@ extern "C" int mcrt_measure_peaks(int device, uint64_t copy_bytes, double *copy_
         return best;
     };
     if (hipMalloc(&src, n * sizeof(float4)) != hipSuccess || hipMalloc(&dst, n * sizeof(float4)) != hipSuccess ||
-        hipMalloc(&sink, (size_t)mfma_blocks * 256u * sizeof(float)) != hipSuccess) {
+        hipMalloc(&sink, (size_t)valu_blocks * 256u * sizeof(float)) != hipSuccess) {
         rc = MCRT_ERR_OOM;
     } else if (hipEventCreate(&ev.a) != hipSuccess || hipEventCreate(&ev.b) != hipSuccess) {
         rc = MCRT_ERR_HIP;
@@ -136,6 +161,12 @@ extern "C" int mcrt_measure_peaks(int device, uint64_t copy_bytes, double *copy_
             hipLaunchKernelGGL(k_mfma_probe<16>, dim3(mfma_blocks), dim3(256), 0, nullptr, iters, 1e-3f, sink);
         }, 3);
         if (rc == MCRT_OK) *mfma_bf16_16x16x32_tflops = waves * iters * kMfmaChains * 16384.0 / (ms * 1e-3) / 1e12;
+        const uint32_t viters = 20000;
+        ms = best_ms([&] {
+            hipLaunchKernelGGL(k_valu_probe, dim3(valu_blocks), dim3(256), 0, nullptr, viters, 1e-3f, sink);
+        }, 3);
+        if (rc == MCRT_OK)
+            *valu_fp32_tflops = (double)valu_blocks * 256.0 * viters * kValuChains * 2.0 / (ms * 1e-3) / 1e12;
     }
     if (src) (void)hipFree(src);
     if (dst) (void)hipFree(dst);

@@ -134,21 +134,24 @@ struct ExtOccIO {
     }
 };
 
-// Like k_extend, the sphere-free plain-stack variant runs 8 waves per SIMD
-// (64 VGPRs, no spill; the 1080p icosphere frame with next-event estimation:
-// 1 spp 3.98 -> 3.88 ms, 4 spp 13.05 -> 11.96 ms against the compiler's 7).
+// Like k_extend, the sphere-free variants run 8 waves per SIMD (64 VGPRs, no
+// spill; the 1080p icosphere frame with next-event estimation: 1 spp 3.98 ->
+// 3.88 ms, 4 spp 13.05 -> 11.96 ms against the compiler's 7).
 template <int kBlock, bool kSpheres, bool kCull>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu((kSpheres || kCull) ? 1 : 8, 8)))
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kSpheres ? 1 : 8, 8)))
 void k_extend_occ(ExtArgs a) {
     __shared__ uint32_t lds[TStack<kCull>::kShort * TStack<kCull>::kWords * kBlock];
     trace_persistent<kBlock, kSpheres, kCull>(a.scene, ExtOccIO{a, *a.count}, lds, a.ovf, a.ovf_stride);
 }
 
-// The sphere-free, plain-stack variant (cache-resident scenes) fits 64 VGPRs
-// and runs 8 waves per SIMD (icosphere Cornell: +1.2% over 7); the others keep the
-// compiler's choice (forcing 8 there spills, or costs cache on large scenes).
+// The sphere-free variants fit 64 VGPRs and run 8 waves per SIMD: the plain
+// stack (cache-resident scenes; icosphere Cornell +1.2% over 7) and the
+// culling stack over quantized pairs (voxel terrain 1236 -> 1266
+// Msamples/s; with 64-B pairs, 8 waves had cost it 3%: more rays in flight
+// competed for the cache).  The sphere variants keep the compiler's choice
+// (forcing 8 there spills).
 template <int kBlock, bool kSpheres, bool kCull>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu((kSpheres || kCull) ? 1 : 8, 8)))
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kSpheres ? 1 : 8, 8)))
 void k_extend(ExtArgs a) {
     __shared__ uint32_t lds[TStack<kCull>::kShort * TStack<kCull>::kWords * kBlock];
     trace_persistent<kBlock, kSpheres, kCull>(a.scene, ExtIO<!kSpheres && !kCull>{a}, lds, a.ovf, a.ovf_stride);

@@ -245,7 +245,7 @@ extern "C" {
     pub fn mcrt_reset_stats(ctx: *mut McrtCtx) -> c_int;
     pub fn mcrt_render_counters(ctx: *mut McrtCtx, segments: *mut u64, samples: *mut u64) -> c_int;
     pub fn mcrt_measure_peaks(device: c_int, copy_bytes: u64, copy_gbs: *mut f64, mfma_bf16_tflops: *mut f64,
-                              mfma_bf16_16x16x32_tflops: *mut f64) -> c_int;
+                              mfma_bf16_16x16x32_tflops: *mut f64, valu_fp32_tflops: *mut f64) -> c_int;
 }
 
 // ---------------------------------------------------------------------------

@@ -16,8 +16,10 @@ import bench  # noqa: E402
 
 def test_kernel_fractions_closed_forms(monkeypatch):
     # A copy of exactly 1 GiB that FETCH_SIZE counts at half (the part's
-    # behaviour) gives factors 2 and 1; a kernel that issues every cycle on
-    # every SIMD with all lanes active has VALU issue 1 and lane efficiency 1.
+    # behaviour) gives factors 2 and 1.  The FP32 fma probe issuing one wave64
+    # instruction per 2 cycles on every SIMD calibrates 2 cycles per
+    # instruction; a kernel issuing at that rate with all lanes active then has
+    # an instruction rate of 1 against the probe and lane efficiency 1.
     class Props:
         multi_processor_count = 256
 
@@ -28,9 +30,10 @@ def test_kernel_fractions_closed_forms(monkeypatch):
     cycles = 1_000_000
     counters = {
         bench.COPY_KERNEL: {"FETCH_SIZE": (gib_kib / 2, 6), "WRITE_SIZE": (gib_kib, 6)},
+        bench.VALU_KERNEL: {"GRBM_GUI_ACTIVE": (8.0 * 3000, 4), "SQ_INSTS_VALU": (1024 * 3000 / 2.0, 4)},
         "k": {"FETCH_SIZE": (1000.0, 1), "WRITE_SIZE": (500.0, 1), "TCC_HIT_sum": (3.0, 1),
               "TCC_MISS_sum": (1.0, 1), "GRBM_GUI_ACTIVE": (8.0 * cycles, 1),
-              "SQ_INSTS_VALU": (1024 * cycles / 4.0, 1), "SQ_THREAD_CYCLES_VALU": (64 * 1024 * cycles / 4.0, 1),
+              "SQ_INSTS_VALU": (1024 * cycles / 2.0, 1), "SQ_THREAD_CYCLES_VALU": (64 * 1024 * cycles / 2.0, 1),
               "TA_BUSY_avr": (cycles / 2.0, 1), "SQ_INSTS_VMEM_RD": (7.0, 1)},
     }
     f = bench.traffic_factors(counters)
@@ -39,8 +42,13 @@ def test_kernel_fractions_closed_forms(monkeypatch):
     assert r["traffic_bytes"] == round(1000 * 1024 * 2 + 500 * 1024)
     assert r["hbm_gbs"] == pytest.approx(r["traffic_bytes"] / 1e-3 / 1e9, abs=0.05)  # rounded to 0.1
     assert r["l2_hit_rate"] == 0.75
-    assert r["valu_issue_frac"] == 1.0 and r["lane_efficiency"] == 1.0 and r["valu_lane_frac"] == 1.0
+    assert r["valu_cycles_per_inst_probe"] == 2.0
+    assert r["valu_inst_rate_vs_fma_probe"] == 1.0 and r["lane_efficiency"] == 1.0
     assert r["ta_busy_frac"] == 0.5 and r["cycles_per_xcd"] == cycles
+    # without the probe's counters no issue fraction is claimed
+    del counters[bench.VALU_KERNEL]
+    r = bench.kernel_fractions(counters, "k", avg_ms=1.0)
+    assert "valu_inst_rate_vs_fma_probe" not in r and r["lane_efficiency"] == 1.0
 
 
 def test_missing_copy_calibration_gives_no_traffic():
