@@ -460,6 +460,10 @@ def roofline(ctx, w, step, segments, counters, peaks):
         res["counters"] = kf
         res["counters_source"] = ("rocprofv3 --pmc passes of `bench.py --probe` in this invocation: "
                                   + " | ".join(" ".join(p) for p in PMC_PASSES))
+        res["binding"] = ("not HBM bandwidth: the bytes each node step fetches through the vector-memory path "
+                          "(counters.ta_busy_frac) and the latency of dependent node fetches over lanes that "
+                          "diverge (counters.lane_efficiency); added to every node step, one 16-B load costs "
+                          "the launch 13% and 24 VALU instructions 1.4%")
     return res
 
 
