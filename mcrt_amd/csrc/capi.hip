@@ -204,8 +204,11 @@ int drain_profile(mcrt_ctx *c) {
 }
 
 // --- scene conversion -------------------------------------------------------
+constexpr uint32_t kTopNodes = 64;  // device slots of the breadth-first top levels
+
 struct TreeOut {
     std::vector<float4> nodes;  // 4 per internal node
+    uint32_t top = 0;           // slots [0, top) hold the breadth-first top levels
     uint32_t root = kNone;
     float lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
     uint32_t depth = 1;
@@ -276,10 +279,38 @@ int convert_tree(mcrt_ctx *c, const char *what, const GpuBvhNode *nodes, uint32_
     // Subtrees are laid out depth-first.  Unused slots stay zero and are never
     // referenced.
     uint32_t n_dev = 0;
+    out.top = 0;
     if (nodes[0].prim_count == 0) {
         dev_index[0] = 0;
         n_dev = 2;  // the root alone in the first line
-        std::vector<uint32_t> todo{0u};
+        // The top levels breadth-first, as far as kTopNodes device slots go
+        // (the traversal keeps them in LDS), then each remaining subtree
+        // depth-first.
+        std::vector<uint32_t> frontier, q{0u};
+        for (size_t h = 0; h < q.size(); ++h) {
+            const uint32_t i = q[h];
+            const uint32_t a = i + 1, b = nodes[i].right_or_offset;
+            const bool ia = nodes[a].prim_count == 0, ib = nodes[b].prim_count == 0;
+            const uint32_t need = (ia && ib) ? 2u + (n_dev & 1u) : (ia || ib) ? 1u : 0u;
+            if (!frontier.empty() || n_dev + need > kTopNodes) {
+                frontier.push_back(i);
+                continue;
+            }
+            if (ia && ib) {
+                n_dev += n_dev & 1u;
+                dev_index[a] = n_dev;
+                dev_index[b] = n_dev + 1;
+                n_dev += 2;
+            } else if (ia) {
+                dev_index[a] = n_dev++;
+            } else if (ib) {
+                dev_index[b] = n_dev++;
+            }
+            if (ia) q.push_back(a);
+            if (ib) q.push_back(b);
+        }
+        out.top = n_dev;
+        std::vector<uint32_t> todo(frontier.rbegin(), frontier.rend());
         while (!todo.empty()) {
             const uint32_t i = todo.back();
             todo.pop_back();
@@ -700,6 +731,7 @@ extern "C" int mcrt_upload_scene(mcrt_ctx *c, const GpuSphere *spheres, uint32_t
     // shorter LDS stack cost more than they save (icosphere Cornell: -3.5%).
     // The culling-stack kernels read the quantized pairs.
     d.stack_cull = quant ? 1u : 0u;
+    d.mtop = quant ? 0u : std::min<uint32_t>(mt.top, kNodeCache);
     if (d.stack_depth > 96)
         return fail(c, MCRT_ERR_LIMIT, "BVH depth %u exceeds the traversal stack limit of 96",
                     d.stack_depth);

@@ -34,6 +34,7 @@ namespace mcrt {
 // A child reference is a device node index, or kLeafBit | first primitive
 // index in leaf order.  A leaf's primitives run until the one whose
 // kPrimLast flag is set.
+constexpr uint32_t kNodeCache = 64;  // 64-B child pairs of the top levels in LDS (4 KB per block)
 constexpr uint32_t kLeafBit = 0x80000000u;
 constexpr uint32_t kNone = 0xFFFFFFFFu;
 constexpr uint32_t kPrimLast = 1u;
@@ -71,6 +72,7 @@ struct DevScene {
     uint32_t env_w, env_h;
     uint32_t stack_depth;     // traversal stack entries needed per lane
     uint32_t refill;          // idle lanes (1..64) at which a traversal wave loads new rays
+    uint32_t mtop;            // plain-stack kernels: mesh nodes [0, mtop) are kept in LDS (<= kNodeCache)
     uint32_t stack_cull;      // traversal kernels with the culling stack (TStack<true>), and
                               // quantized 32-B child pairs in mnodes / snodes (node_quant.h)
     // Emitters for next-event estimation: emissive triangles and spheres,
@@ -310,7 +312,7 @@ __device__ __forceinline__ uint32_t enter_root(const DevScene &sc, bool sphere, 
 template <bool kQuant, class S>
 __device__ __forceinline__ void node_step(const float4 *nodes, V3 o, V3 inv, float t_min,
                                           float t_max, float best_t, uint32_t &cur, uint32_t &sp,
-                                          const S &st) {
+                                          const S &st, const float4 *cache = nullptr, uint32_t ncache = 0) {
     float te0, tx0, te1, tx1;
     uint32_t c0, c1;
     if constexpr (kQuant) {
@@ -326,8 +328,24 @@ __device__ __forceinline__ void node_step(const float4 *nodes, V3 o, V3 inv, flo
         c0 = wb.z;
         c1 = wb.w;
     } else {
-        const float4 *n = nodes + 4u * cur;
-        float4 q0 = n[0], q1 = n[1], q2 = n[2], q3 = n[3];
+        typedef float nf4 __attribute__((ext_vector_type(4)));
+        nf4 r0, r1, r2, r3;
+        if (cur < ncache) {
+            const __attribute__((address_space(3))) nf4 *n =
+                (const __attribute__((address_space(3))) nf4 *)cache + 4u * cur;
+            r0 = n[0];
+            r1 = n[1];
+            r2 = n[2];
+            r3 = n[3];
+        } else {
+            const __attribute__((address_space(1))) nf4 *n = (const __attribute__((address_space(1))) nf4 *)nodes + 4u * cur;
+            r0 = n[0];
+            r1 = n[1];
+            r2 = n[2];
+            r3 = n[3];
+        }
+        const float4 q0 = make_float4(r0.x, r0.y, r0.z, r0.w), q1 = make_float4(r1.x, r1.y, r1.z, r1.w);
+        const float4 q2 = make_float4(r2.x, r2.y, r2.z, r2.w), q3 = make_float4(r3.x, r3.y, r3.z, r3.w);
         slab2(o, inv, q0.x, q0.y, q0.z, q0.w, q2.x, q2.y, te0, tx0);
         slab2(o, inv, q1.x, q1.y, q1.z, q1.w, q2.z, q2.w, te1, tx1);
         c0 = __float_as_uint(q3.x);
@@ -484,7 +502,14 @@ __device__ __forceinline__ Best trace_one(const DevScene &sc, V3 o, V3 d, float 
 // the culling stack (lds holds TStack<kCull>::kShort * kWords words per lane).
 template <int kBlock, bool kSpheres, bool kCull, class IO>
 __device__ __forceinline__ void trace_persistent(const DevScene &sc, const IO &io, uint32_t *lds,
-                                                 uint32_t *ovf, uint32_t ovf_stride) {
+                                                 uint32_t *ovf, uint32_t ovf_stride,
+                                                 float4 *cache = nullptr) {
+    uint32_t ncache = 0;
+    if (!kCull && cache) {  // the breadth-first top levels of the mesh tree
+        ncache = sc.mtop;
+        for (uint32_t i = threadIdx.x; i < 4u * ncache; i += kBlock) cache[i] = sc.mnodes[i];
+        __syncthreads();
+    }
     const uint32_t lane = __lane_id();
     TStack<kCull> st{lds, ovf, ovf_stride, threadIdx.x, blockIdx.x * kBlock + threadIdx.x, (uint32_t)kBlock};
     const uint32_t total = io.count();
@@ -561,7 +586,8 @@ __device__ __forceinline__ void trace_persistent(const DevScene &sc, const IO &i
             if (__ballot(in && leaf == kNone) == 0) break;
             if (in) {
                 const float4 *nodes = (kSpheres && sphere_phase) ? sc.snodes : sc.mnodes;
-                node_step<kCull>(nodes, o, inv, t_min, t_cur, best.t, cur, sp, st);
+                node_step<kCull>(nodes, o, inv, t_min, t_cur, best.t, cur, sp, st, cache,
+                                 (kSpheres && sphere_phase) ? 0u : ncache);
             }
         }
         // ---- leaves ----------------------------------------------------------

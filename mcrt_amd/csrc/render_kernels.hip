@@ -141,7 +141,8 @@ template <int kBlock, bool kSpheres, bool kCull>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kSpheres ? 1 : 8, 8)))
 void k_extend_occ(ExtArgs a) {
     __shared__ uint32_t lds[TStack<kCull>::kShort * TStack<kCull>::kWords * kBlock];
-    trace_persistent<kBlock, kSpheres, kCull>(a.scene, ExtOccIO{a, *a.count}, lds, a.ovf, a.ovf_stride);
+    __shared__ float4 ncache[kCull ? 1 : kNodeCache * 4];
+    trace_persistent<kBlock, kSpheres, kCull>(a.scene, ExtOccIO{a, *a.count}, lds, a.ovf, a.ovf_stride, kCull ? nullptr : ncache);
 }
 
 // The sphere-free variants fit 64 VGPRs and run 8 waves per SIMD: the plain
@@ -154,7 +155,8 @@ template <int kBlock, bool kSpheres, bool kCull>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kSpheres ? 1 : 8, 8)))
 void k_extend(ExtArgs a) {
     __shared__ uint32_t lds[TStack<kCull>::kShort * TStack<kCull>::kWords * kBlock];
-    trace_persistent<kBlock, kSpheres, kCull>(a.scene, ExtIO<!kSpheres && !kCull>{a}, lds, a.ovf, a.ovf_stride);
+    __shared__ float4 ncache[kCull ? 1 : kNodeCache * 4];
+    trace_persistent<kBlock, kSpheres, kCull>(a.scene, ExtIO<!kSpheres && !kCull>{a}, lds, a.ovf, a.ovf_stride, kCull ? nullptr : ncache);
 }
 
 // ---------------------------------------------------------------------------
@@ -334,7 +336,8 @@ struct OccIO {
 template <int kBlock, bool kSpheres, bool kCull>
 __global__ __launch_bounds__(kBlock) void k_occlude(OccArgs a) {
     __shared__ uint32_t lds[TStack<kCull>::kShort * TStack<kCull>::kWords * kBlock];
-    trace_persistent<kBlock, kSpheres, kCull>(a.scene, OccIO{a}, lds, a.ovf, a.ovf_stride);
+    __shared__ float4 ncache[kCull ? 1 : kNodeCache * 4];
+    trace_persistent<kBlock, kSpheres, kCull>(a.scene, OccIO{a}, lds, a.ovf, a.ovf_stride, kCull ? nullptr : ncache);
 }
 
 // ---------------------------------------------------------------------------
@@ -382,7 +385,8 @@ struct TraceIO {
 template <int kBlock, bool kSpheres, bool kCull>
 __global__ __launch_bounds__(kBlock) void k_trace(TraceArgs a) {
     __shared__ uint32_t lds[TStack<kCull>::kShort * TStack<kCull>::kWords * kBlock];
-    trace_persistent<kBlock, kSpheres, kCull>(a.scene, TraceIO{a}, lds, a.ovf, a.ovf_stride);
+    __shared__ float4 ncache[kCull ? 1 : kNodeCache * 4];
+    trace_persistent<kBlock, kSpheres, kCull>(a.scene, TraceIO{a}, lds, a.ovf, a.ovf_stride, kCull ? nullptr : ncache);
 }
 
 __global__ void k_fill_u32(uint32_t *p, uint32_t n, uint32_t v) {
