@@ -20,6 +20,7 @@
 #include <vector>
 
 #include "../../include/mcrt_abi.h"
+#include "bvh_refine.h"
 #include "kernels.h"
 
 using namespace mcrt;
@@ -552,6 +553,28 @@ extern "C" int mcrt_upload_scene(mcrt_ctx *c, const GpuSphere *spheres, uint32_t
     TreeOut mt, st;
     rc = convert_tree(c, "mesh", mesh_bvh, n_mesh_nodes, n_tris, mt);
     if (rc) return rc;
+    // Leaves of at most 2 triangles: fewer triangle tests and shorter
+    // dependent leaf loops for about 3% more node steps.
+    std::vector<GpuBvhNode> rnodes;
+    std::vector<uint32_t> perm, rorig;
+    std::vector<GpuTriangle> rtris;
+    if (refine_mesh_tree(mesh_bvh, n_mesh_nodes, verts, tris, n_tris, rnodes, perm) &&
+        rnodes.size() < 0xFFFFFFFFu) {
+        TreeOut rt;
+        rc = convert_tree(c, "mesh", rnodes.data(), (uint32_t)rnodes.size(), n_tris, rt);
+        if (rc) return rc;
+        if (rt.depth <= 96u || rt.depth <= mt.depth) {
+            mt = std::move(rt);
+            rtris.resize(n_tris);
+            rorig.resize(n_tris);
+            for (uint32_t j = 0; j < n_tris; ++j) {
+                rtris[j] = tris[perm[j]];
+                rorig[j] = tri_orig_index ? tri_orig_index[perm[j]] : perm[j];
+            }
+            tris = rtris.data();
+            tri_orig_index = rorig.data();
+        }
+    }
     rc = convert_tree(c, "sphere", sphere_bvh, n_sphere_nodes, n_spheres, st);
     if (rc) return rc;
     // Bytes a traversal can touch (nodes + triangles + spheres); the per-scene
