@@ -161,6 +161,7 @@ const char *mcrt_last_error(const mcrt_ctx *ctx);
  *  - sphere_orig_index: likewise for spheres; NULL = identity.
  *  - Empty sets may be passed as n = 0 or as the reference's stub entries
  *    (STUB_SPHERE, zeroed node, src/gpu_layout.rs:264-312).
+ *  - At most 2^30 - 1 triangles and 2^30 - 1 spheres (MCRT_ERR_LIMIT).
  *  - env_rgba32f: env_w x env_h RGBA32F equirectangular map, row 0 = zenith
  *    (src/texture.rs:158-220).  Required.
  *  - albedo_rgba8: n_layers x size x size RGBA8 sRGB texture array

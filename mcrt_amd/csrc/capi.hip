@@ -222,7 +222,7 @@ bool is_stub_tree(const GpuBvhNode *nodes, uint32_t n) {
 
 // Validates a reference pre-order tree and converts it to child pairs.
 int convert_tree(mcrt_ctx *c, const char *what, const GpuBvhNode *nodes, uint32_t n,
-                 uint32_t n_prims, TreeOut &out) {
+                 uint32_t n_prims, TreeOut &out, bool pair_leaves = false) {
     out.last.assign(n_prims, 0);
     if (is_stub_tree(nodes, n)) return MCRT_OK;
     if (!nodes) return fail(c, MCRT_ERR_BAD_ARG, "%s BVH: null node array", what);
@@ -332,7 +332,9 @@ int convert_tree(mcrt_ctx *c, const char *what, const GpuBvhNode *nodes, uint32_
         }
     }
     auto ref = [&](uint32_t i) -> uint32_t {
-        return nodes[i].prim_count == 0 ? dev_index[i] : (kLeafBit | nodes[i].right_or_offset);
+        if (nodes[i].prim_count == 0) return dev_index[i];
+        const bool pair = pair_leaves && nodes[i].prim_count == 2 && n_prims < kLeafPair;
+        return kLeafBit | (pair ? kLeafPair : 0u) | nodes[i].right_or_offset;
     };
     out.nodes.assign((size_t)std::max<uint32_t>(n_dev, 1) * 4, make_float4(0, 0, 0, 0));
     for (uint32_t i = 0; i < n; ++i) {
@@ -533,6 +535,9 @@ extern "C" int mcrt_upload_scene(mcrt_ctx *c, const GpuSphere *spheres, uint32_t
         return fail(c, MCRT_ERR_BAD_ARG, "environment map is required (got %ux%u)", env_w, env_h);
     if ((n_spheres && !spheres) || (n_tris && (!tris || !verts)))
         return fail(c, MCRT_ERR_BAD_ARG, "null primitive array with a non-zero count");
+    if (n_tris >= kLeafPair || n_spheres >= kLeafPair)  // primitive indices share a word with two flag bits
+        return fail(c, MCRT_ERR_LIMIT, "at most %u triangles and %u spheres (got %u and %u)", kLeafPair - 1u,
+                    kLeafPair - 1u, n_tris, n_spheres);
     if (n_layers > 0 && albedo_rgba8) {
         for (uint32_t m = 0; m < MCRT_MAX_MATERIALS; ++m) {
             const GpuMaterial &g = materials->materials[m];
@@ -551,7 +556,7 @@ extern "C" int mcrt_upload_scene(mcrt_ctx *c, const GpuSphere *spheres, uint32_t
                             tris[i].v[k], n_verts);
 
     TreeOut mt, st;
-    rc = convert_tree(c, "mesh", mesh_bvh, n_mesh_nodes, n_tris, mt);
+    rc = convert_tree(c, "mesh", mesh_bvh, n_mesh_nodes, n_tris, mt, true);
     if (rc) return rc;
     // Leaves of at most 2 triangles: fewer triangle tests and shorter
     // dependent leaf loops for about 3% more node steps.
@@ -561,7 +566,7 @@ extern "C" int mcrt_upload_scene(mcrt_ctx *c, const GpuSphere *spheres, uint32_t
     if (refine_mesh_tree(mesh_bvh, n_mesh_nodes, verts, tris, n_tris, rnodes, perm) &&
         rnodes.size() < 0xFFFFFFFFu) {
         TreeOut rt;
-        rc = convert_tree(c, "mesh", rnodes.data(), (uint32_t)rnodes.size(), n_tris, rt);
+        rc = convert_tree(c, "mesh", rnodes.data(), (uint32_t)rnodes.size(), n_tris, rt, true);
         if (rc) return rc;
         if (rt.depth <= 96u || rt.depth <= mt.depth) {
             mt = std::move(rt);
@@ -694,6 +699,24 @@ extern "C" int mcrt_upload_scene(mcrt_ctx *c, const GpuSphere *spheres, uint32_t
     // encode keeps the 64-B pairs and the plain stack.
     std::vector<uint32_t> qm, qs;
     const bool quant = !fits_l2 && quantize_tree(mt.nodes, qm) && quantize_tree(st.nodes, qs);
+    // Leaves of exactly 2 triangles fetch both records at once in the
+    // culling-stack kernels (the 1080p voxel terrain: +1.7%); the plain-stack
+    // kernels of cache-resident scenes lost 1.5% with it (the icosphere
+    // Cornell scene), so their trees drop the kLeafPair flag.
+    if (!quant) {
+        auto strip = [](uint32_t r) { return (r != kNone && (r & kLeafBit)) ? (r & ~kLeafPair) : r; };
+        for (size_t i = 0; i < mt.nodes.size() / 4; ++i) {
+            float4 &q = mt.nodes[4 * i + 3];
+            uint32_t r0, r1;
+            std::memcpy(&r0, &q.x, 4);
+            std::memcpy(&r1, &q.y, 4);
+            r0 = strip(r0);
+            r1 = strip(r1);
+            std::memcpy(&q.x, &r0, 4);
+            std::memcpy(&q.y, &r1, 4);
+        }
+        mt.root = strip(mt.root);
+    }
     if (quant) {
         if ((rc = upload(c, c->mnodes, qm.data(), qm.size() * sizeof(uint32_t)))) return rc;
     } else {

@@ -33,9 +33,11 @@ namespace mcrt {
 // node step, boxes that contain the reference boxes.
 // A child reference is a device node index, or kLeafBit | first primitive
 // index in leaf order.  A leaf's primitives run until the one whose
-// kPrimLast flag is set.
+// kPrimLast flag is set; in the mesh tree kLeafPair also marks a leaf of
+// exactly 2 triangles, whose records are then fetched together.
 constexpr uint32_t kNodeCache = 64;  // 64-B child pairs of the top levels in LDS (4 KB per block)
 constexpr uint32_t kLeafBit = 0x80000000u;
+constexpr uint32_t kLeafPair = 0x40000000u;
 constexpr uint32_t kNone = 0xFFFFFFFFu;
 constexpr uint32_t kPrimLast = 1u;
 constexpr uint32_t kPrimCull = 2u;
@@ -376,32 +378,44 @@ __device__ __forceinline__ void node_step(const float4 *nodes, V3 o, V3 inv, flo
     }
 }
 
-__device__ __forceinline__ void tri_leaf(const DevScene &sc, uint32_t i, V3 o, V3 d, float t_min,
+__device__ __forceinline__ void tri_test(const DevScene &sc, uint32_t i, float4 a, float4 b, float4 c, V3 o,
+                                         V3 d, float t_min, float t_max, Best &best) {
+    float t, u, v;
+    if (mt_hit(o, d, xyz(a), xyz(b), xyz(c), t_min, t_max, t, u, v)) {
+        uint32_t id = __float_as_uint(a.w);
+        if (best.prim == kNone || better(t, id, best)) {
+            bool ok = true;
+            if (__float_as_uint(b.w) & kPrimCull) {  // :438-452, culled candidates do not shrink t
+                uint32_t mat;
+                V3 nn = tri_normal(sc, i, u, v, mat);
+                ok = dot(d, nn) < 0.0f;
+            }
+            if (ok) {
+                best.t = t;
+                best.id = id;
+                best.prim = i;
+                best.u = u;
+                best.v = v;
+            }
+        }
+    }
+}
+
+// Leaf at primitive i; pair: exactly 2 triangles, both records fetched at once.
+__device__ __forceinline__ void tri_leaf(const DevScene &sc, uint32_t i, bool pair, V3 o, V3 d, float t_min,
                                          float t_max, Best &best) {
+    if (pair) {
+        const float4 *tp = sc.mtris + sc.tri_stride * i, *tq = tp + sc.tri_stride;
+        const float4 a0 = tp[0], b0 = tp[1], c0 = tp[2], a1 = tq[0], b1 = tq[1], c1 = tq[2];
+        tri_test(sc, i, a0, b0, c0, o, d, t_min, t_max, best);
+        tri_test(sc, i + 1u, a1, b1, c1, o, d, t_min, t_max, best);
+        return;
+    }
     for (;;) {
         const float4 *tp = sc.mtris + sc.tri_stride * i;
         float4 a = tp[0], b = tp[1], c = tp[2];
-        uint32_t flags = __float_as_uint(b.w);
-        float t, u, v;
-        if (mt_hit(o, d, xyz(a), xyz(b), xyz(c), t_min, t_max, t, u, v)) {
-            uint32_t id = __float_as_uint(a.w);
-            if (best.prim == kNone || better(t, id, best)) {
-                bool ok = true;
-                if (flags & kPrimCull) {  // :438-452, culled candidates do not shrink t
-                    uint32_t mat;
-                    V3 nn = tri_normal(sc, i, u, v, mat);
-                    ok = dot(d, nn) < 0.0f;
-                }
-                if (ok) {
-                    best.t = t;
-                    best.id = id;
-                    best.prim = i;
-                    best.u = u;
-                    best.v = v;
-                }
-            }
-        }
-        if (flags & kPrimLast) break;
+        tri_test(sc, i, a, b, c, o, d, t_min, t_max, best);
+        if (__float_as_uint(b.w) & kPrimLast) break;
         ++i;
     }
 }
@@ -469,7 +483,7 @@ __device__ __forceinline__ Best trace_one(const DevScene &sc, V3 o, V3 d, float 
             if (kSpheres && sphere_phase)
                 sphere_leaf(sc, leaf & ~kLeafBit, o, d, t_min, t_cur, best);
             else
-                tri_leaf(sc, leaf & ~kLeafBit, o, d, t_min, t_cur, best);
+                tri_leaf(sc, leaf & ~(kLeafBit | kLeafPair), (leaf & kLeafPair) != 0u, o, d, t_min, t_cur, best);
             leaf = kNone;
         }
         if (!done && cur == kNone && leaf == kNone) {
@@ -595,7 +609,7 @@ __device__ __forceinline__ void trace_persistent(const DevScene &sc, const IO &i
             if (kSpheres && sphere_phase)
                 sphere_leaf(sc, leaf & ~kLeafBit, o, d, t_min, t_cur, best);
             else
-                tri_leaf(sc, leaf & ~kLeafBit, o, d, t_min, t_cur, best);
+                tri_leaf(sc, leaf & ~(kLeafBit | kLeafPair), (leaf & kLeafPair) != 0u, o, d, t_min, t_cur, best);
             leaf = kNone;
             if (IO::kAnyHit && any && best.prim != kNone) {  // a visibility query is answered
                 cur = kNone;
