@@ -216,9 +216,10 @@ def main():
         "mrays_per_s": round(segments / sec / 1e6, 3),
         "segments_per_step": int(segments),
     }
-    relmse_c3 = relmse_windows(w, accum, denoised, "c3_oracle_b.npz")
-    if relmse_c3:
-        out["relmse_vs_oracle_b"] = relmse_c3
+    if w.name == "voxel_c3":  # the oracle-B windows are of the voxel terrain frame
+        relmse_c3 = relmse_windows(w, accum, denoised, "c3_oracle_b.npz")
+        if relmse_c3:
+            out["relmse_vs_oracle_b"] = relmse_c3
 
     lines = {}
     if not a.no_lines:
