@@ -430,9 +430,9 @@ def _main(argv=None):
     sub = ap.add_subparsers(dest="cmd", required=True)
     t = sub.add_parser("train")
     t.add_argument("--variants", type=int, default=512)
-    t.add_argument("--spp-in", default="4", help="input sample count, or several separated by commas")
+    t.add_argument("--spp-in", default="2,4,8,16", help="input sample count, or several separated by commas")
     t.add_argument("--spp-ref", type=int, default=4096)
-    t.add_argument("--steps", type=int, default=16000)
+    t.add_argument("--steps", type=int, default=192000)
     t.add_argument("--batch", type=int, default=16)
     t.add_argument("--lr", type=float, default=1e-3)
     t.add_argument("--seed", type=int, default=0)
