@@ -980,7 +980,10 @@ extern "C" int mcrt_render_aux_dev(mcrt_ctx *c, const CameraUniform *cam, uint32
                     e.samples = (float4 *)c->samples.p;
                 }
                 Timed t(c, "extend", hs[h]);
-                HIP_TRY(c, launch_extend(e, g_ext, hs[h]));
+                if (b == 0 && c->scene.stack_cull && c->scene.sroot == kNone && !e.occ_count)
+                    HIP_TRY(c, launch_extend_packet(e, g_ext, hs[h]));
+                else
+                    HIP_TRY(c, launch_extend(e, g_ext, hs[h]));
             }
             for (uint32_t h = 0; h < nh; ++h) {
                 const uint32_t np = pn[h] * fn[h];

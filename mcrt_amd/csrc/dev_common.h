@@ -637,4 +637,135 @@ __device__ __forceinline__ void trace_persistent(const DevScene &sc, const IO &i
     }
 }
 
+// ---------------------------------------------------------------------------
+// Packet traversal for coherent queues (first-bounce camera rays: a wave's
+// 64 slots are 4 neighbouring pixels x 16 frames).  The wave walks one
+// wave-uniform path through the tree: a child is entered when any lane's
+// slab test passes it with the lane's own cull limit, so every lane visits a
+// superset of the nodes its own traversal visits and keeps the lowest
+// (t, original id); results equal the per-lane traversal's.  Nodes and
+// triangles are wave-uniform and come in through scalar loads (no
+// vector-memory address work); the stack is wave-uniform and held one entry
+// per lane in two register pairs (128 entries; trees are at most 96 deep).
+// Each stack entry keeps the smallest entry distance over the lanes that
+// hit the far child, so a pop is skipped when no lane's limit reaches it.
+// Quantized pairs only (scenes on the culling-stack kernels), sphere-free.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t wave_umin(uint32_t v) {
+    for (int s = 32; s >= 1; s >>= 1) v = min(v, (uint32_t)__shfl_xor((int)v, s));
+    return v;
+}
+
+template <int kBlock, class IO>
+__device__ __forceinline__ void trace_packets(const DevScene &sc, const IO &io) {
+    typedef uint32_t pu4 __attribute__((ext_vector_type(4)));
+    typedef float pf4 __attribute__((ext_vector_type(4)));
+    const __attribute__((address_space(4))) pu4 *qn = (const __attribute__((address_space(4))) pu4 *)sc.mnodes;
+    const __attribute__((address_space(4))) pf4 *tr = (const __attribute__((address_space(4))) pf4 *)sc.mtris;
+    const uint32_t lane = __lane_id();
+    const uint32_t total = io.count();
+    const uint32_t waves = gridDim.x * (uint32_t)(kBlock / 64);
+    const uint32_t wave_id = blockIdx.x * (uint32_t)(kBlock / 64) + threadIdx.x / 64u;
+    const uint32_t first = min(kChunk, (total + waves - 1u) / waves);
+    const uint32_t dyn0 = first * waves;
+    uint32_t cbase = min(wave_id * first, total), cend = min(cbase + first, total);
+    bool more = dyn0 < total;
+    for (;;) {
+        if (cbase >= cend) {
+            if (!more) break;
+            uint32_t g = 0;
+            if (lane == 0) g = dyn0 + atomicAdd(io.work(), kChunk);
+            g = __builtin_amdgcn_readfirstlane(g);
+            if (g >= total) break;
+            cbase = g;
+            cend = min(g + kChunk, total);
+        }
+        const uint32_t slot = cbase + lane;
+        const bool valid = slot < cend;
+        cbase = min(cbase + 64u, cend);
+        V3 o = v3(0, 0, 0), d = v3(1, 1, 1);
+        float t_min = 0.0f, t_max = 0.0f;
+        if (valid) io.load(slot, o, d, t_min, t_max);
+        const V3 inv = v3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+        Best best;
+        best.t = t_max;
+        best.id = best.prim = kNone;
+        best.u = best.v = 0.0f;
+        const bool act = valid && enter_root(sc, false, o, inv, t_min, t_max) != kNone;
+        uint32_t cur = __ballot(act) ? sc.mroot : kNone;
+        uint32_t sp = 0;
+        uint32_t sr0 = 0, sr1 = 0, st0 = 0, st1 = 0;  // stack entry k in lane k % 64 of pair k / 64
+        while (cur != kNone) {
+            if (!(cur & kLeafBit)) {
+                const pu4 wa = qn[2u * cur], wb = qn[2u * cur + 1u];
+                const float sx = qnode_step(wa.x), ox = qnode_origin(wa.x);
+                const float sy = qnode_step(wa.y), oy = qnode_origin(wa.y);
+                const float sz = qnode_step(wa.z), oz = qnode_origin(wa.z);
+                float te0, tx0, te1, tx1;
+                slab2(o, inv, qnode_plane(wa.w, 0, sx, ox), qnode_plane(wa.w, 1, sx, ox), qnode_plane(wa.w, 2, sy, oy),
+                      qnode_plane(wa.w, 3, sy, oy), qnode_plane(wb.x, 0, sz, oz), qnode_plane(wb.x, 1, sz, oz), te0, tx0);
+                slab2(o, inv, qnode_plane(wb.x, 2, sx, ox), qnode_plane(wb.x, 3, sx, ox), qnode_plane(wb.y, 0, sy, oy),
+                      qnode_plane(wb.y, 1, sy, oy), qnode_plane(wb.y, 2, sz, oz), qnode_plane(wb.y, 3, sz, oz), te1, tx1);
+                const float lim = best.t * kCullSlack;
+                const float bound = fminf(t_max, __uint_as_float(__float_as_uint(lim) + 1u));
+                const bool h0 = act && tx0 > fmaxf(te0, t_min) && te0 < bound;
+                const bool h1 = act && tx1 > fmaxf(te1, t_min) && te1 < bound;
+                const uint64_t m0 = __ballot(h0), m1 = __ballot(h1);
+                const uint32_t c0 = wb.z, c1 = wb.w;
+                if (m0 && m1) {
+                    const uint64_t both = m0 & m1;
+                    const uint64_t first1 = __ballot(h0 && h1 && te1 < te0);
+                    const bool near1 = both ? 2u * (uint32_t)__popcll(first1) > (uint32_t)__popcll(both)
+                                            : __popcll(m1) > __popcll(m0);
+                    const bool hf = near1 ? h0 : h1;
+                    const float tef = fmaxf(near1 ? te0 : te1, t_min);
+                    const uint32_t fte = __builtin_amdgcn_readfirstlane(wave_umin(hf ? __float_as_uint(tef) : 0x7F800000u));
+                    const uint32_t far = near1 ? c0 : c1;
+                    if (sp < 64u) {
+                        sr0 = lane == sp ? far : sr0;
+                        st0 = lane == sp ? fte : st0;
+                    } else {
+                        sr1 = lane == sp - 64u ? far : sr1;
+                        st1 = lane == sp - 64u ? fte : st1;
+                    }
+                    ++sp;
+                    cur = near1 ? c1 : c0;
+                    continue;
+                }
+                if (m0) {
+                    cur = c0;
+                    continue;
+                }
+                if (m1) {
+                    cur = c1;
+                    continue;
+                }
+            } else {
+                uint32_t i = cur & ~(kLeafBit | kLeafPair);
+                const bool pair = (cur & kLeafPair) != 0u;
+                for (uint32_t k = 0;; ++k, ++i) {
+                    i = __builtin_amdgcn_readfirstlane(i);  // wave-uniform: scalar loads
+                    const __attribute__((address_space(4))) pf4 *tp = tr + sc.tri_stride * i;
+                    const pf4 a = tp[0], b = tp[1], c = tp[2];
+                    if (act)
+                        tri_test(sc, i, make_float4(a.x, a.y, a.z, a.w), make_float4(b.x, b.y, b.z, b.w),
+                                 make_float4(c.x, c.y, c.z, c.w), o, d, t_min, t_max, best);
+                    if (pair ? k == 1u : (__float_as_uint(b.w) & kPrimLast) != 0u) break;
+                }
+            }
+            cur = kNone;
+            while (sp > 0u) {
+                --sp;
+                const uint32_t r = sp < 64u ? __builtin_amdgcn_readlane(sr0, sp) : __builtin_amdgcn_readlane(sr1, sp - 64u);
+                const uint32_t te = sp < 64u ? __builtin_amdgcn_readlane(st0, sp) : __builtin_amdgcn_readlane(st1, sp - 64u);
+                if (__ballot(act && __uint_as_float(te) <= best.t * kCullSlack)) {
+                    cur = r;
+                    break;
+                }
+            }
+        }
+        if (valid) io.store(slot, best);
+    }
+}
+
 }  // namespace mcrt

@@ -361,6 +361,13 @@ __global__ __launch_bounds__(256) void k_accumulate(AccArgs a) {
     a.accum_out[pix] = make_float4(acc.x, acc.y, acc.z, 1.0f);
 }
 
+// k_extend_packet: first bounce of a sphere-free scene on the culling-stack
+// kernels, traced as wave-uniform packets (trace_packets).
+template <int kBlock>
+__global__ __launch_bounds__(kBlock) void k_extend_packet(ExtArgs a) {
+    trace_packets<kBlock>(a.scene, ExtIO<false>{a});
+}
+
 // ---------------------------------------------------------------------------
 // k_trace: standalone closest-hit query (mcrt_trace_closest)
 // ---------------------------------------------------------------------------
@@ -440,6 +447,10 @@ hipError_t launch_traversal(const void *fn, const Args &a, uint32_t grid, hipStr
     Args copy = a;
     void *args[] = {&copy};
     return hipLaunchKernel(fn, dim3(grid), dim3(kTraceBlock), args, 0, s);
+}
+
+hipError_t launch_extend_packet(const ExtArgs &a, uint32_t grid, hipStream_t s) {
+    return launch_traversal(reinterpret_cast<const void *>(&k_extend_packet<kTraceBlock>), a, grid, s);
 }
 
 hipError_t launch_extend(const ExtArgs &a, uint32_t grid, hipStream_t s) {

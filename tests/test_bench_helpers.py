@@ -74,3 +74,17 @@ def test_more_than_one_gpu_is_refused():
     r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
                        text=True, timeout=120)
     assert r.returncode == 2 and "one GPU" in r.stderr and r.stdout == ""
+
+
+def test_c3_extension_counters_average_over_both_kernels():
+    """One C3 render runs the first bounce as k_extend_packet and bounces 1-5
+    as k_extend<256, false, true>: the step's per-launch means weigh every
+    dispatch of both; C2's kernel stays apart."""
+    means = {
+        "void mcrt::k_extend_packet<256>(mcrt::ExtArgs)": {"FETCH_SIZE": (100.0, 1)},
+        "void mcrt::k_extend<256, false, true>(mcrt::ExtArgs)": {"FETCH_SIZE": (400.0, 5)},
+        "void mcrt::k_extend<256, false, false>(mcrt::ExtArgs)": {"FETCH_SIZE": (7.0, 16)},
+    }
+    g = bench.group_counter_means(means)
+    assert g[bench.EXTEND_C3]["FETCH_SIZE"] == ((100.0 + 5 * 400.0) / 6, 6)
+    assert g[bench.EXTEND_C2]["FETCH_SIZE"] == (7.0, 16)
