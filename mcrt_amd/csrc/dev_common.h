@@ -59,6 +59,7 @@ constexpr uint32_t kPrimCull = 2u;
 
 struct DevScene {
     const float4 *mnodes;
+    const float4 *mnodes_full;  // quantized scenes: the same pairs at 64 B (packet traversal), else null
     const float4 *mtris;
     const float4 *tshade;
     uint32_t tri_stride;     // float4 per triangle record: 3, or 4 (padded) for scenes that spill the L2
@@ -645,11 +646,13 @@ __device__ __forceinline__ void trace_persistent(const DevScene &sc, const IO &i
 // superset of the nodes its own traversal visits and keeps the lowest
 // (t, original id); results equal the per-lane traversal's.  Nodes and
 // triangles are wave-uniform and come in through scalar loads (no
-// vector-memory address work); the stack is wave-uniform and held one entry
+// vector-memory address work): a node is the exact 64-B pair, one
+// s_load_dwordx16, with no plane decode (DevScene::mnodes_full, uploaded
+// beside the quantized pairs).  The stack is wave-uniform and held one entry
 // per lane in two register pairs (128 entries; trees are at most 96 deep).
 // Each stack entry keeps the smallest entry distance over the lanes that
 // hit the far child, so a pop is skipped when no lane's limit reaches it.
-// Quantized pairs only (scenes on the culling-stack kernels), sphere-free.
+// Scenes on the culling-stack kernels, sphere-free.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ uint32_t wave_umin(uint32_t v) {
     for (int s = 32; s >= 1; s >>= 1) v = min(v, (uint32_t)__shfl_xor((int)v, s));
@@ -658,9 +661,8 @@ __device__ __forceinline__ uint32_t wave_umin(uint32_t v) {
 
 template <int kBlock, class IO>
 __device__ __forceinline__ void trace_packets(const DevScene &sc, const IO &io) {
-    typedef uint32_t pu4 __attribute__((ext_vector_type(4)));
     typedef float pf4 __attribute__((ext_vector_type(4)));
-    const __attribute__((address_space(4))) pu4 *qn = (const __attribute__((address_space(4))) pu4 *)sc.mnodes;
+    const __attribute__((address_space(4))) pf4 *nf = (const __attribute__((address_space(4))) pf4 *)sc.mnodes_full;
     const __attribute__((address_space(4))) pf4 *tr = (const __attribute__((address_space(4))) pf4 *)sc.mtris;
     const uint32_t lane = __lane_id();
     const uint32_t total = io.count();
@@ -697,21 +699,17 @@ __device__ __forceinline__ void trace_packets(const DevScene &sc, const IO &io) 
         uint32_t sr0 = 0, sr1 = 0, st0 = 0, st1 = 0;  // stack entry k in lane k % 64 of pair k / 64
         while (cur != kNone) {
             if (!(cur & kLeafBit)) {
-                const pu4 wa = qn[2u * cur], wb = qn[2u * cur + 1u];
-                const float sx = qnode_step(wa.x), ox = qnode_origin(wa.x);
-                const float sy = qnode_step(wa.y), oy = qnode_origin(wa.y);
-                const float sz = qnode_step(wa.z), oz = qnode_origin(wa.z);
+                const __attribute__((address_space(4))) pf4 *qf = nf + 4u * cur;
+                const pf4 q0 = qf[0], q1 = qf[1], q2 = qf[2], q3 = qf[3];
                 float te0, tx0, te1, tx1;
-                slab2(o, inv, qnode_plane(wa.w, 0, sx, ox), qnode_plane(wa.w, 1, sx, ox), qnode_plane(wa.w, 2, sy, oy),
-                      qnode_plane(wa.w, 3, sy, oy), qnode_plane(wb.x, 0, sz, oz), qnode_plane(wb.x, 1, sz, oz), te0, tx0);
-                slab2(o, inv, qnode_plane(wb.x, 2, sx, ox), qnode_plane(wb.x, 3, sx, ox), qnode_plane(wb.y, 0, sy, oy),
-                      qnode_plane(wb.y, 1, sy, oy), qnode_plane(wb.y, 2, sz, oz), qnode_plane(wb.y, 3, sz, oz), te1, tx1);
+                slab2(o, inv, q0.x, q0.y, q0.z, q0.w, q2.x, q2.y, te0, tx0);
+                slab2(o, inv, q1.x, q1.y, q1.z, q1.w, q2.z, q2.w, te1, tx1);
                 const float lim = best.t * kCullSlack;
                 const float bound = fminf(t_max, __uint_as_float(__float_as_uint(lim) + 1u));
                 const bool h0 = act && tx0 > fmaxf(te0, t_min) && te0 < bound;
                 const bool h1 = act && tx1 > fmaxf(te1, t_min) && te1 < bound;
                 const uint64_t m0 = __ballot(h0), m1 = __ballot(h1);
-                const uint32_t c0 = wb.z, c1 = wb.w;
+                const uint32_t c0 = __float_as_uint(q3.x), c1 = __float_as_uint(q3.y);
                 if (m0 && m1) {
                     const uint64_t both = m0 & m1;
                     const uint64_t first1 = __ballot(h0 && h1 && te1 < te0);
