@@ -751,7 +751,7 @@ extern "C" int mcrt_upload_scene(mcrt_ctx *c, const GpuSphere *spheres, uint32_t
     d.tri_emit = d.n_emit ? (const uint32_t *)c->tri_emit.p : nullptr;
     d.sph_emit = d.n_emit ? (const uint32_t *)c->sph_emit.p : nullptr;
     d.mnodes = (const float4 *)c->mnodes.p;
-    d.mnodes_full = quant ? (const float4 *)c->mnodes_full.p : nullptr;
+    d.mnodes_full = (const float4 *)(quant ? c->mnodes_full.p : c->mnodes.p);
     d.mtris = (const float4 *)c->mtris.p;
     d.tshade = (const float4 *)c->tshade.p;
     d.tshade_stride = ts;
@@ -984,7 +984,7 @@ extern "C" int mcrt_render_aux_dev(mcrt_ctx *c, const CameraUniform *cam, uint32
                     e.samples = (float4 *)c->samples.p;
                 }
                 Timed t(c, "extend", hs[h]);
-                if (b == 0 && c->scene.stack_cull && c->scene.mnodes_full && c->scene.sroot == kNone && !e.occ_count)
+                if (b == 0 && c->scene.sroot == kNone && !e.occ_count)
                     HIP_TRY(c, launch_extend_packet(e, g_ext, hs[h]));
                 else
                     HIP_TRY(c, launch_extend(e, g_ext, hs[h]));

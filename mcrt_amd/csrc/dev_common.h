@@ -59,7 +59,7 @@ constexpr uint32_t kPrimCull = 2u;
 
 struct DevScene {
     const float4 *mnodes;
-    const float4 *mnodes_full;  // quantized scenes: the same pairs at 64 B (packet traversal), else null
+    const float4 *mnodes_full;  // the pairs at 64 B (packet traversal): mnodes, or a copy beside quantized ones
     const float4 *mtris;
     const float4 *tshade;
     uint32_t tri_stride;     // float4 per triangle record: 3, or 4 (padded) for scenes that spill the L2
@@ -647,12 +647,12 @@ __device__ __forceinline__ void trace_persistent(const DevScene &sc, const IO &i
 // (t, original id); results equal the per-lane traversal's.  Nodes and
 // triangles are wave-uniform and come in through scalar loads (no
 // vector-memory address work): a node is the exact 64-B pair, one
-// s_load_dwordx16, with no plane decode (DevScene::mnodes_full, uploaded
-// beside the quantized pairs).  The stack is wave-uniform and held one entry
+// s_load_dwordx16, with no plane decode (DevScene::mnodes_full; scenes with
+// quantized pairs upload the 64-B pairs beside them).  The stack is wave-uniform and held one entry
 // per lane in two register pairs (128 entries; trees are at most 96 deep).
 // Each stack entry keeps the smallest entry distance over the lanes that
 // hit the far child, so a pop is skipped when no lane's limit reaches it.
-// Scenes on the culling-stack kernels, sphere-free.
+// Sphere-free scenes, first bounce.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ uint32_t wave_umin(uint32_t v) {
     for (int s = 32; s >= 1; s >>= 1) v = min(v, (uint32_t)__shfl_xor((int)v, s));

@@ -212,7 +212,7 @@ hipError_t launch_unpack(const float4 *out_log, float4 *out, size_t n, hipStream
 hipError_t launch_fill_u32(uint32_t *p, uint32_t n, uint32_t v, hipStream_t s);
 hipError_t launch_generate(const GenArgs &a, hipStream_t s);
 hipError_t launch_extend(const ExtArgs &a, uint32_t grid, hipStream_t s);
-// The first bounce of a sphere-free scene whose nodes are quantized pairs.
+// The first bounce of a sphere-free scene, as wave-uniform packets.
 hipError_t launch_extend_packet(const ExtArgs &a, uint32_t grid, hipStream_t s);
 hipError_t launch_shade(const ShadeArgs &a, uint32_t grid, hipStream_t s);
 hipError_t launch_accumulate(const AccArgs &a, hipStream_t s);

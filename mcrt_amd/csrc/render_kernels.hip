@@ -361,11 +361,12 @@ __global__ __launch_bounds__(256) void k_accumulate(AccArgs a) {
     a.accum_out[pix] = make_float4(acc.x, acc.y, acc.z, 1.0f);
 }
 
-// k_extend_packet: first bounce of a sphere-free scene on the culling-stack
-// kernels, traced as wave-uniform packets (trace_packets).
-template <int kBlock>
+// k_extend_packet: first bounce of a sphere-free scene, traced as
+// wave-uniform packets (trace_packets).  kGen: the launch recomputes pinhole
+// camera rays (a.gen.on) instead of reading them.
+template <int kBlock, bool kGen>
 __global__ __launch_bounds__(kBlock) void k_extend_packet(ExtArgs a) {
-    trace_packets<kBlock>(a.scene, ExtIO<false>{a});
+    trace_packets<kBlock>(a.scene, ExtIO<kGen>{a});
 }
 
 // ---------------------------------------------------------------------------
@@ -450,7 +451,9 @@ hipError_t launch_traversal(const void *fn, const Args &a, uint32_t grid, hipStr
 }
 
 hipError_t launch_extend_packet(const ExtArgs &a, uint32_t grid, hipStream_t s) {
-    return launch_traversal(reinterpret_cast<const void *>(&k_extend_packet<kTraceBlock>), a, grid, s);
+    const void *fn = a.gen.on ? reinterpret_cast<const void *>(&k_extend_packet<kTraceBlock, true>)
+                              : reinterpret_cast<const void *>(&k_extend_packet<kTraceBlock, false>);
+    return launch_traversal(fn, a, grid, s);
 }
 
 hipError_t launch_extend(const ExtArgs &a, uint32_t grid, hipStream_t s) {
