@@ -52,8 +52,9 @@ MFMA_BF16_PEAK_TFLOPS = 2500.0  # dense BF16, published
 N_XCD = 8
 COPY_PROBE_BYTES = 1 << 30
 EXTEND_C3 = "k_extend<256, false, true>"   # culling stack (the terrain spills the L2), bounces 1+
-EXTEND_C3_FIRST = "k_extend_packet<256>"   # the same scene's first bounce, wave-uniform packets
-EXTEND_C2 = "k_extend<256, false, false>"  # plain stack (cache-resident scene)
+EXTEND_C3_FIRST = "k_extend_packet<256, false>"  # the same scene's first bounce, wave-uniform packets
+EXTEND_C2 = "k_extend<256, false, false>"  # plain stack (cache-resident scene), bounces 1+
+EXTEND_C2_FIRST = "k_extend_packet<256, true>"   # its first bounce: packets, camera rays computed in the kernel
 COPY_KERNEL = "k_copy_probe"
 VALU_KERNEL = "k_valu_probe"
 PMC_PASSES = [
@@ -322,10 +323,10 @@ def counter_passes(w, keep_dir=None):
 
 def group_counter_means(means):
     """Per-dispatch means by kernel name -> by the short names the bench
-    reports.  The C3 extension step is two kernels (first bounce, later
+    reports.  An extension step is two kernels (first bounce, later
     bounces); its means are taken over the dispatches of both, as the event
     timing of the step is."""
-    groups = {EXTEND_C3: (EXTEND_C3, EXTEND_C3_FIRST), EXTEND_C2: (EXTEND_C2,), COPY_KERNEL: (COPY_KERNEL,),
+    groups = {EXTEND_C3: (EXTEND_C3, EXTEND_C3_FIRST), EXTEND_C2: (EXTEND_C2, EXTEND_C2_FIRST), COPY_KERNEL: (COPY_KERNEL,),
               VALU_KERNEL: (VALU_KERNEL,), "k_shade": ("k_shade",)}
     by_short = {}
     for short, names in groups.items():
@@ -474,13 +475,18 @@ def roofline(ctx, w, step, segments, counters, peaks):
             res["frac"] = kf["hbm_frac"]
             res["frac_of_measured_copy"] = round(kf["hbm_gbs"] / peaks["hbm_copy_gbs"], 4)
             res["bytes_per_segment"] = round(kf["traffic_bytes"] / max(segments / max(launches, 1), 1), 1)
+        # The probe renders one step: one extension dispatch per bounce (C3).
+        n = counters[EXTEND_C3 if w.name == "voxel_c3" else EXTEND_C2].get("FETCH_SIZE", (0.0, 0))[1]
+        kf["dispatches"] = n
+        if w.name == "voxel_c3" and n != w.depth:
+            kf["dispatches_note"] = f"expected {w.depth} extension dispatches per render, counted {n}"
         res["counters"] = kf
         res["counters_source"] = ("rocprofv3 --pmc passes of `bench.py --probe` in this invocation: "
                                   + " | ".join(" ".join(p) for p in PMC_PASSES))
-        res["binding"] = ("not HBM bandwidth: the bytes each node step fetches through the vector-memory path "
+        res["binding"] = ("not HBM bandwidth: the loads each node step issues through the vector-memory path "
                           "(counters.ta_busy_frac) and the latency of dependent node fetches over lanes that "
-                          "diverge (counters.lane_efficiency); added to every node step, one 16-B load costs "
-                          "the launch 13% and 24 VALU instructions 1.4%")
+                          "diverge (counters.lane_efficiency); added to every node step, one more 16-B load "
+                          "costs the launch 8.5%, while 24 more VALU instructions cost 1.4%")
     return res
 
 

@@ -77,14 +77,16 @@ def test_more_than_one_gpu_is_refused():
 
 
 def test_c3_extension_counters_average_over_both_kernels():
-    """One C3 render runs the first bounce as k_extend_packet and bounces 1-5
-    as k_extend<256, false, true>: the step's per-launch means weigh every
-    dispatch of both; C2's kernel stays apart."""
+    """A render runs the first bounce as k_extend_packet and the later bounces
+    as k_extend: the step's per-launch means weigh every dispatch of both
+    (C3: <256, false> + <256, false, true>; C2: <256, true> + <256, false,
+    false>), and the two scenes stay apart."""
     means = {
-        "void mcrt::k_extend_packet<256>(mcrt::ExtArgs)": {"FETCH_SIZE": (100.0, 1)},
+        "void mcrt::k_extend_packet<256, false>(mcrt::ExtArgs)": {"FETCH_SIZE": (100.0, 1)},
         "void mcrt::k_extend<256, false, true>(mcrt::ExtArgs)": {"FETCH_SIZE": (400.0, 5)},
-        "void mcrt::k_extend<256, false, false>(mcrt::ExtArgs)": {"FETCH_SIZE": (7.0, 16)},
+        "void mcrt::k_extend_packet<256, true>(mcrt::ExtArgs)": {"FETCH_SIZE": (3.0, 2)},
+        "void mcrt::k_extend<256, false, false>(mcrt::ExtArgs)": {"FETCH_SIZE": (7.0, 14)},
     }
     g = bench.group_counter_means(means)
     assert g[bench.EXTEND_C3]["FETCH_SIZE"] == ((100.0 + 5 * 400.0) / 6, 6)
-    assert g[bench.EXTEND_C2]["FETCH_SIZE"] == (7.0, 16)
+    assert g[bench.EXTEND_C2]["FETCH_SIZE"] == ((2 * 3.0 + 14 * 7.0) / 16, 16)
