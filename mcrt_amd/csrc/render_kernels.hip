@@ -65,10 +65,9 @@ __global__ __launch_bounds__(256) void k_generate(GenArgs a) {
 // k_extend: closest hit of every queued path (world_hit at [T_MIN, T_MAX))
 // ---------------------------------------------------------------------------
 // kGen: the launch may recompute first-bounce camera rays (a.gen.on).  Only
-// the sphere-free plain-stack kernel (cache-resident scenes) does: the camera
-// arithmetic at the refill point costs registers, which the culling-stack
-// kernel of large scenes cannot spare (1080p voxel terrain -2%, against +2%
-// on the icosphere Cornell scene).
+// the first-bounce packet kernel of cache-resident, sphere-free scenes does
+// (k_extend_packet<.., true>); the per-lane kernels read stored rays (the
+// camera arithmetic at their refill point cost the 1080p voxel terrain 2%).
 template <bool kGen>
 struct ExtIO {
     const ExtArgs &a;
@@ -156,7 +155,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kSpheres
 void k_extend(ExtArgs a) {
     __shared__ uint32_t lds[TStack<kCull>::kShort * TStack<kCull>::kWords * kBlock];
     __shared__ float4 ncache[kCull ? 1 : kNodeCache * 4];
-    trace_persistent<kBlock, kSpheres, kCull>(a.scene, ExtIO<!kSpheres && !kCull>{a}, lds, a.ovf, a.ovf_stride, kCull ? nullptr : ncache);
+    trace_persistent<kBlock, kSpheres, kCull>(a.scene, ExtIO<false>{a}, lds, a.ovf, a.ovf_stride, kCull ? nullptr : ncache);
 }
 
 // ---------------------------------------------------------------------------
