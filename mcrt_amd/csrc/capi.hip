@@ -698,11 +698,49 @@ extern "C" int mcrt_upload_scene(mcrt_ctx *c, const GpuSphere *spheres, uint32_t
     // the 16M-ray closest-hit launch).  A tree with a box the grid cannot
     // encode keeps the 64-B pairs and the plain stack.
     std::vector<uint32_t> qm, qs;
+    // Quad leaves (scenes that spill the L2, 64-B triangle records): a leaf of
+    // two triangles (a, b, c) + (a, c, d), in either order, keeps both in its
+    // first record (dev_common.h tri_leaf); other 2-triangle leaves lose the
+    // kLeafPair flag and take the per-record loop.  Equality is bitwise.
+    if (!fits_l2 && tsd == 4u) {
+        auto same3 = [](const float4 &x, const float4 &y) {
+            return std::memcmp(&x, &y, 3 * sizeof(float)) == 0;
+        };
+        auto quad = [&](uint32_t r) -> uint32_t {
+            if (r == kNone || !(r & kLeafBit) || !(r & kLeafPair)) return r;
+            const uint32_t i = r & ~(kLeafBit | kLeafPair);
+            float4 *x = &dt[4 * (size_t)i], *y = &dt[4 * (size_t)i + 4];
+            if (!same3(x[0], y[0])) return r & ~kLeafPair;
+            uint32_t fl;
+            if (same3(y[1], x[2])) {  // x = (a, b, c), y = (a, c, d)
+                std::memcpy(&fl, &y[1].w, 4);
+                x[3] = make_float4(y[2].x, y[2].y, y[2].z, ubits(fl));
+            } else if (same3(x[1], y[2])) {  // x = (a, c, d), y = (a, b, c)
+                std::memcpy(&fl, &y[1].w, 4);
+                x[3] = make_float4(y[1].x, y[1].y, y[1].z, ubits(fl | kQuadSwap));
+            } else {
+                return r & ~kLeafPair;
+            }
+            x[2].w = y[0].w;  // the second triangle's id
+            return r;
+        };
+        for (size_t n = 0; n < mt.nodes.size() / 4; ++n) {
+            float4 &q = mt.nodes[4 * n + 3];
+            uint32_t r0, r1;
+            std::memcpy(&r0, &q.x, 4);
+            std::memcpy(&r1, &q.y, 4);
+            r0 = quad(r0);
+            r1 = quad(r1);
+            std::memcpy(&q.x, &r0, 4);
+            std::memcpy(&q.y, &r1, 4);
+        }
+        mt.root = quad(mt.root);
+    }
     const bool quant = !fits_l2 && quantize_tree(mt.nodes, qm) && quantize_tree(st.nodes, qs);
-    // Leaves of exactly 2 triangles fetch both records at once in the
-    // culling-stack kernels (the 1080p voxel terrain: +1.7%); the plain-stack
-    // kernels of cache-resident scenes lost 1.5% with it (the icosphere
-    // Cornell scene), so their trees drop the kLeafPair flag.
+    // Quad leaves serve the culling-stack kernels only (the 1080p voxel
+    // terrain: +1.7% for fetching a leaf's two records at once, +1.7% more
+    // for one 64-B quad record); the plain-stack kernels of cache-resident
+    // scenes compile no quad path, so their trees drop the kLeafPair flag.
     if (!quant) {
         auto strip = [](uint32_t r) { return (r != kNone && (r & kLeafBit)) ? (r & ~kLeafPair) : r; };
         for (size_t i = 0; i < mt.nodes.size() / 4; ++i) {

@@ -33,14 +33,16 @@ namespace mcrt {
 // node step, boxes that contain the reference boxes.
 // A child reference is a device node index, or kLeafBit | first primitive
 // index in leaf order.  A leaf's primitives run until the one whose
-// kPrimLast flag is set; in the mesh tree kLeafPair also marks a leaf of
-// exactly 2 triangles, whose records are then fetched together.
+// kPrimLast flag is set; in the mesh tree of a scene on the culling-stack
+// kernels, kLeafPair also marks a quad leaf (tri_leaf): two triangles that
+// share a vertex and an edge, both held in the first 64-B record.
 constexpr uint32_t kNodeCache = 64;  // 64-B child pairs of the top levels in LDS (4 KB per block)
 constexpr uint32_t kLeafBit = 0x80000000u;
 constexpr uint32_t kLeafPair = 0x40000000u;
 constexpr uint32_t kNone = 0xFFFFFFFFu;
 constexpr uint32_t kPrimLast = 1u;
 constexpr uint32_t kPrimCull = 2u;
+constexpr uint32_t kQuadSwap = 4u;  // quad record: the leaf's first triangle is (a, c, d), the second (a, b, c)
 
 // Triangle (leaf order, DevScene::tri_stride float4 apart: 48 B for scenes
 // that fit the L2, 64 B with a padding float4 otherwise), used by traversal:
@@ -402,13 +404,25 @@ __device__ __forceinline__ void tri_test(const DevScene &sc, uint32_t i, float4 
     }
 }
 
-// Leaf at primitive i; pair: exactly 2 triangles, both records fetched at once.
+// Leaf at primitive i.  pair: the leaf is a quad, two triangles (a, b, c)
+// and (a, c, d) in either order, and the first record (64 B) holds both:
+// its spare slots carry the second triangle's id (third float4, w) and its
+// remaining edge and flags (fourth float4).  The shared edge c - a is the
+// same FP32 value in both triangles' own records, so both tests see exactly
+// their own records' operands, from 4 loads instead of 6.  kQuads = false
+// compiles without that path (kernels of scenes whose trees never set
+// kLeafPair: it cost their registers).
+template <bool kQuads = true>
 __device__ __forceinline__ void tri_leaf(const DevScene &sc, uint32_t i, bool pair, V3 o, V3 d, float t_min,
                                          float t_max, Best &best) {
-    if (pair) {
-        const float4 *tp = sc.mtris + sc.tri_stride * i, *tq = tp + sc.tri_stride;
-        const float4 a0 = tp[0], b0 = tp[1], c0 = tp[2], a1 = tq[0], b1 = tq[1], c1 = tq[2];
-        tri_test(sc, i, a0, b0, c0, o, d, t_min, t_max, best);
+    if (kQuads && pair) {
+        const float4 *tp = sc.mtris + sc.tri_stride * i;
+        const float4 a = tp[0], b = tp[1], c = tp[2], q = tp[3];
+        tri_test(sc, i, a, b, c, o, d, t_min, t_max, best);
+        const bool sw = (__float_as_uint(q.w) & kQuadSwap) != 0u;
+        const float4 a1 = make_float4(a.x, a.y, a.z, c.w);
+        const float4 b1 = sw ? q : make_float4(c.x, c.y, c.z, q.w);
+        const float4 c1 = sw ? b : q;
         tri_test(sc, i + 1u, a1, b1, c1, o, d, t_min, t_max, best);
         return;
     }
@@ -610,7 +624,7 @@ __device__ __forceinline__ void trace_persistent(const DevScene &sc, const IO &i
             if (kSpheres && sphere_phase)
                 sphere_leaf(sc, leaf & ~kLeafBit, o, d, t_min, t_cur, best);
             else
-                tri_leaf(sc, leaf & ~(kLeafBit | kLeafPair), (leaf & kLeafPair) != 0u, o, d, t_min, t_cur, best);
+                tri_leaf<kCull>(sc, leaf & ~(kLeafBit | kLeafPair), (leaf & kLeafPair) != 0u, o, d, t_min, t_cur, best);
             leaf = kNone;
             if (IO::kAnyHit && any && best.prim != kNone) {  // a visibility query is answered
                 cur = kNone;
