@@ -21,6 +21,7 @@
 
 #include "../../include/mcrt_abi.h"
 #include "bvh_refine.h"
+#include "bvh_wide.h"
 #include "kernels.h"
 
 using namespace mcrt;
@@ -74,7 +75,7 @@ struct mcrt_ctx {
     hipStream_t last_stream = nullptr;
     // scene
     bool has_scene = false;
-    DevBuf mnodes, mnodes_full, mtris, tshade, snodes, sph, mats, env;
+    DevBuf mnodes, mnodes_full, mwide, mtris, tshade, snodes, sph, mats, env;
     DevBuf emit, emit_cdf, tri_emit, sph_emit;  // next-event estimation emitters
     DevBuf ovf, ovf2;  // traversal-stack overflow for trees deeper than the LDS short stack (per stream)
     hipStream_t side_stream = nullptr;  // second stream of a split batch (default priority)
@@ -491,7 +492,7 @@ extern "C" void mcrt_destroy(mcrt_ctx *c) {
     (void)hipStreamSynchronize(c->last_stream);
     (void)hipStreamSynchronize(c->own_stream);
     if (c->side_stream) (void)hipStreamSynchronize(c->side_stream);
-    for (DevBuf *b : {&c->mnodes, &c->mnodes_full, &c->mtris, &c->tshade, &c->ovf, &c->ovf2, &c->snodes, &c->sph, &c->mats,
+    for (DevBuf *b : {&c->mnodes, &c->mnodes_full, &c->mwide, &c->mtris, &c->tshade, &c->ovf, &c->ovf2, &c->snodes, &c->sph, &c->mats,
                       &c->env, &c->A[0], &c->A[1], &c->B[0], &c->B[1], &c->C[0], &c->C[1], &c->H,
                       &c->samples, &c->counters, &c->host_accum, &c->host_gbuf, &c->host_out,
                       &c->host_accum_in, &c->trace_o, &c->trace_d, &c->trace_id, &c->trace_t,
@@ -563,6 +564,10 @@ extern "C" int mcrt_upload_scene(mcrt_ctx *c, const GpuSphere *spheres, uint32_t
     std::vector<GpuBvhNode> rnodes;
     std::vector<uint32_t> perm, rorig;
     std::vector<GpuTriangle> rtris;
+    // given[j]: the index in the given triangle order of leaf-order record j
+    // (the emitter list keeps the given order whatever the records' order).
+    std::vector<uint32_t> given(n_tris);
+    for (uint32_t j = 0; j < n_tris; ++j) given[j] = j;
     if (refine_mesh_tree(mesh_bvh, n_mesh_nodes, verts, tris, n_tris, rnodes, perm) &&
         rnodes.size() < 0xFFFFFFFFu) {
         TreeOut rt;
@@ -578,6 +583,7 @@ extern "C" int mcrt_upload_scene(mcrt_ctx *c, const GpuSphere *spheres, uint32_t
             }
             tris = rtris.data();
             tri_orig_index = rorig.data();
+            given = perm;
         }
     }
     rc = convert_tree(c, "sphere", sphere_bvh, n_sphere_nodes, n_spheres, st);
@@ -587,6 +593,48 @@ extern "C" int mcrt_upload_scene(mcrt_ctx *c, const GpuSphere *spheres, uint32_t
     const size_t traversal_bytes = (mt.nodes.size() + st.nodes.size()) * sizeof(float4) +
                                    (size_t)n_tris * 48 + (size_t)n_spheres * 32;
     const bool fits_l2 = traversal_bytes <= (32u << 20);
+
+    // Scenes that spill the L2: the culling-stack kernels traverse the mesh
+    // tree as 4-wide 48-B nodes (bvh_wide.h).  The leaf records are
+    // reordered here, before any per-record array is built, so that the
+    // leaves of each wide node follow one another; the binary tree's leaf
+    // references follow the new order.
+    WidePlan wplan;
+    bool wide = false;
+    std::vector<GpuTriangle> wtris;
+    std::vector<uint32_t> worig;
+    if (!fits_l2 && n_tris && mt.root != kNone && !(mt.root & kLeafBit)) {
+        const WidePairs bp{mt.nodes.data(), mt.nodes.size() / 4, kLeafBit, kLeafPair, kNone};
+        std::vector<uint32_t> wperm, wstart;
+        if (wide_plan(bp, mt.root, wplan) && wplan.stack_need <= 96u &&
+            wide_leaf_order(bp, wplan, mt.last.data(), n_tris, wperm, wstart)) {
+            wtris.resize(n_tris);
+            worig.resize(n_tris);
+            std::vector<uint8_t> wlast(n_tris);
+            std::vector<uint32_t> wgiven(n_tris);
+            for (uint32_t j = 0; j < n_tris; ++j) {
+                wtris[j] = tris[wperm[j]];
+                worig[j] = tri_orig_index ? tri_orig_index[wperm[j]] : wperm[j];
+                wlast[j] = mt.last[wperm[j]];
+                wgiven[j] = given[wperm[j]];
+            }
+            given = std::move(wgiven);
+            for (size_t i = 0; i < mt.nodes.size() / 4; ++i) {
+                float4 &q = mt.nodes[4 * i + 3];
+                for (float *f : {&q.x, &q.y}) {
+                    uint32_t r;
+                    std::memcpy(&r, f, 4);
+                    if (r == kNone || !(r & kLeafBit)) continue;
+                    r = (r & (kLeafBit | kLeafPair)) | wstart[r & ~(kLeafBit | kLeafPair)];
+                    std::memcpy(f, &r, 4);
+                }
+            }
+            tris = wtris.data();
+            tri_orig_index = worig.data();
+            mt.last = std::move(wlast);
+            wide = true;
+        }
+    }
 
     // Triangles in leaf order: v0, e1 = v1 - v0, e2 = v2 - v0.
     // Scenes that spill the L2 pad each triangle to 64 B, so a record gathered
@@ -632,7 +680,9 @@ extern "C" int mcrt_upload_scene(mcrt_ctx *c, const GpuSphere *spheres, uint32_t
     // Emitters for next-event estimation: triangles and spheres whose material
     // is emissive (emission = colour x strength, :608-610), chosen with
     // probability proportional to area x luminance.  Triangle area from the
-    // same FP32 edges the kernels use.
+    // same FP32 edges the kernels use.  Triangles are listed in the given
+    // order (the order of the caller's triangle array), so a light sample
+    // picks the same triangle whatever order the records take here.
     std::vector<float4> demit;
     std::vector<float> dcdf;
     std::vector<uint32_t> dtri_emit, dsph_emit;
@@ -649,7 +699,10 @@ extern "C" int mcrt_upload_scene(mcrt_ctx *c, const GpuSphere *spheres, uint32_t
             return true;
         };
         auto lum = [](const float le[3]) { return 0.2126f * le[0] + 0.7152f * le[1] + 0.0722f * le[2]; };
-        for (uint32_t i = 0; i < n_tris; ++i) {
+        std::vector<uint32_t> at(n_tris);  // leaf-order record of each given index
+        for (uint32_t j = 0; j < n_tris; ++j) at[given[j]] = j;
+        for (uint32_t g = 0; g < n_tris; ++g) {
+            const uint32_t i = at[g];
             Em e{};
             if (!emission(tris[i].mat_idx & 0x7FFFFFFFu, e.le)) continue;
             const float4 e1 = dt[tsd * (size_t)i + 1], e2 = dt[tsd * (size_t)i + 2];
@@ -736,7 +789,15 @@ extern "C" int mcrt_upload_scene(mcrt_ctx *c, const GpuSphere *spheres, uint32_t
         }
         mt.root = quad(mt.root);
     }
-    const bool quant = !fits_l2 && quantize_tree(mt.nodes, qm) && quantize_tree(st.nodes, qs);
+    bool quant = !fits_l2 && quantize_tree(mt.nodes, qm) && quantize_tree(st.nodes, qs);
+    // The culling-stack kernels read the mesh tree as 4-wide nodes; a mesh
+    // tree with internal nodes but no 4-wide form keeps the plain stack.
+    std::vector<uint32_t> wwords;
+    if (quant && wide) {
+        const WidePairs bp{mt.nodes.data(), mt.nodes.size() / 4, kLeafBit, kLeafPair, kNone};
+        wide = wide_emit(bp, wplan, wwords);
+    }
+    if (quant && n_tris && mt.root != kNone && !(mt.root & kLeafBit) && !wide) quant = false;
     // Quad leaves serve the culling-stack kernels only (the 1080p voxel
     // terrain: +1.7% for fetching a leaf's two records at once, +1.7% more
     // for one 64-B quad record); the plain-stack kernels of cache-resident
@@ -760,6 +821,7 @@ extern "C" int mcrt_upload_scene(mcrt_ctx *c, const GpuSphere *spheres, uint32_t
         // the exact 64-B pairs too, for the first bounce's packet traversal
         // (scalar loads of whole pairs, no plane decode: +2% on the voxel terrain)
         if ((rc = upload(c, c->mnodes_full, mt.nodes.data(), mt.nodes.size() * sizeof(float4)))) return rc;
+        if (wide && (rc = upload(c, c->mwide, wwords.data(), wwords.size() * sizeof(uint32_t)))) return rc;
     } else {
         if ((rc = upload(c, c->mnodes, mt.nodes.data(), mt.nodes.size() * sizeof(float4)))) return rc;
     }
@@ -795,6 +857,8 @@ extern "C" int mcrt_upload_scene(mcrt_ctx *c, const GpuSphere *spheres, uint32_t
     d.tshade_stride = ts;
     d.tri_stride = tsd;
     d.mroot = n_tris ? mt.root : kNone;
+    d.mwide = (quant && wide) ? (const uint4 *)c->mwide.p : nullptr;
+    d.mwroot = !n_tris ? kNone : (quant && wide) ? 0u : mt.root;
     std::memcpy(d.mroot_lo, mt.lo, sizeof mt.lo);
     std::memcpy(d.mroot_hi, mt.hi, sizeof mt.hi);
     d.snodes = (const float4 *)c->snodes.p;
@@ -807,6 +871,7 @@ extern "C" int mcrt_upload_scene(mcrt_ctx *c, const GpuSphere *spheres, uint32_t
     d.env_w = env_w;
     d.env_h = env_h;
     d.stack_depth = std::max(mt.depth, st.depth);
+    if (quant && wide) d.stack_depth = std::max(d.stack_depth, wplan.stack_need);
     // A refill stops the whole wave to load new rays.  Where the traversal
     // set (nodes + triangles + spheres) outgrows the 32 MB of L2 that load's
     // latency is long, and refilling more lanes per stop pays: at 40 idle

@@ -80,6 +80,8 @@ struct DevScene {
     uint32_t mtop;            // plain-stack kernels: mesh nodes [0, mtop) are kept in LDS (<= kNodeCache)
     uint32_t stack_cull;      // traversal kernels with the culling stack (TStack<true>), and
                               // quantized 32-B child pairs in mnodes / snodes (node_quant.h)
+    const uint4 *mwide;       // culling-stack kernels: the mesh tree as 4-wide 48-B nodes (bvh_wide.h)
+    uint32_t mwroot;          // their root: wide node 0, a leaf reference, or kNone
     // Emitters for next-event estimation: emissive triangles and spheres,
     // chosen with probability proportional to area x luminance.
     //   emit[2e] = (prob, kind | leaf-order index bits, area, -),
@@ -299,9 +301,11 @@ using Stack = TStack<false>;
 
 // Root entry of one BVH for a ray: the root box test of the reference
 // (the root node is popped and slab-tested first, :375-379).
+// kWide: the mesh root is that of the 4-wide tree.
+template <bool kWide = false>
 __device__ __forceinline__ uint32_t enter_root(const DevScene &sc, bool sphere, V3 o, V3 inv,
                                                float t_min, float t_max) {
-    const uint32_t root = sphere ? sc.sroot : sc.mroot;
+    const uint32_t root = sphere ? sc.sroot : (kWide ? sc.mwroot : sc.mroot);
     if (root == kNone) return kNone;
     const float *lo = sphere ? sc.sroot_lo : sc.mroot_lo;
     const float *hi = sphere ? sc.sroot_hi : sc.mroot_hi;
@@ -379,6 +383,71 @@ __device__ __forceinline__ void node_step(const float4 *nodes, V3 o, V3 inv, flo
     } else {
         cur = st.pop(sp, lim);
     }
+}
+
+// One 4-wide node step (culling stack; bvh_wide.h layout): slab-test the up
+// to 4 children on their shared grid, descend into the nearest child that
+// passes, and push the others farthest first, each with its entry distance.
+// A child passes where the reference slab test passes on its decoded box
+// (which contains the binary child box it came from) and the entry is within
+// the cull limit, the same rule as node_step.
+template <class S>
+__device__ __forceinline__ void node_step_wide(const uint4 *wide, V3 o, V3 inv, float t_min, float t_max,
+                                               float best_t, uint32_t &cur, uint32_t &sp, const S &st) {
+    const uint4 *n = wide + 3u * cur;
+    const uint4 wa = n[0], wb = n[1], wc = n[2];
+    const float sx = qnode_step(wa.x), ox = qnode_origin(wa.x);
+    const float sy = qnode_step(wa.y), oy = qnode_origin(wa.y);
+    const float sz = qnode_step(wa.z), oz = qnode_origin(wa.z);
+    float te0, tx0, te1, tx1, te2, tx2, te3, tx3;
+    slab2(o, inv, qnode_plane(wa.w, 0, sx, ox), qnode_plane(wa.w, 1, sx, ox), qnode_plane(wa.w, 2, sy, oy),
+          qnode_plane(wa.w, 3, sy, oy), qnode_plane(wb.x, 0, sz, oz), qnode_plane(wb.x, 1, sz, oz), te0, tx0);
+    slab2(o, inv, qnode_plane(wb.x, 2, sx, ox), qnode_plane(wb.x, 3, sx, ox), qnode_plane(wb.y, 0, sy, oy),
+          qnode_plane(wb.y, 1, sy, oy), qnode_plane(wb.y, 2, sz, oz), qnode_plane(wb.y, 3, sz, oz), te1, tx1);
+    slab2(o, inv, qnode_plane(wb.z, 0, sx, ox), qnode_plane(wb.z, 1, sx, ox), qnode_plane(wb.z, 2, sy, oy),
+          qnode_plane(wb.z, 3, sy, oy), qnode_plane(wb.w, 0, sz, oz), qnode_plane(wb.w, 1, sz, oz), te2, tx2);
+    slab2(o, inv, qnode_plane(wb.w, 2, sx, ox), qnode_plane(wb.w, 3, sx, ox), qnode_plane(wc.x, 0, sy, oy),
+          qnode_plane(wc.x, 1, sy, oy), qnode_plane(wc.x, 2, sz, oz), qnode_plane(wc.x, 3, sz, oz), te3, tx3);
+    const float lim = best_t * kCullSlack;
+    const float bound = fminf(t_max, __uint_as_float(__float_as_uint(lim) + 1u));
+    const uint32_t meta = wc.w;
+    // Sort keys: for a passing child, the bits of its entry distance raised
+    // to at least t_min and +0 (non-negative, so the bits order as unsigned
+    // integers), with the slot in the 2 low bits; all ones for the others.
+    // The distance a key carries (low bits cleared) is at most max(entry,
+    // t_min, 0), so a box that holds the closest hit t* (entry <= t*,
+    // t* >= t_min, t* > 0) is never culled by it.  An unused slot is a point
+    // box (every plane byte 0), whose exit never exceeds its entry: it never
+    // passes, with no test of its meta byte (such a test became a branch
+    // around each child, and the node's first 16-B load was split in two).
+    auto key = [&](float te, float tx, uint32_t k) {
+        const bool h = tx > fmaxf(te, t_min) && te < bound;
+        return h ? ((__float_as_uint(fmaxf(fmaxf(te, t_min), 0.0f)) & ~3u) | k) : 0xFFFFFFFFu;
+    };
+    uint32_t k0 = key(te0, tx0, 0), k1 = key(te1, tx1, 1), k2 = key(te2, tx2, 2), k3 = key(te3, tx3, 3);
+    auto cas = [](uint32_t &a, uint32_t &b) {
+        const uint32_t lo = min(a, b), hi = max(a, b);
+        a = lo;
+        b = hi;
+    };
+    cas(k0, k1);
+    cas(k2, k3);
+    cas(k0, k2);
+    cas(k1, k3);
+    cas(k1, k2);
+    auto ref = [&](uint32_t key) {
+        const uint32_t m = (meta >> (8u * (key & 3u))) & 0xFFu;
+        return (m & 0x80u) ? (kLeafBit | ((m & 0x40u) << 24) | (wc.z + (m & 0x3Fu))) : (wc.y + (m & 3u));
+    };
+    auto dist = [](uint32_t key) { return __uint_as_float(key & ~3u); };
+    if (k0 == 0xFFFFFFFFu) {
+        cur = st.pop(sp, lim);
+        return;
+    }
+    if (k3 != 0xFFFFFFFFu) st.set(sp++, ref(k3), dist(k3));
+    if (k2 != 0xFFFFFFFFu) st.set(sp++, ref(k2), dist(k2));
+    if (k1 != 0xFFFFFFFFu) st.set(sp++, ref(k1), dist(k1));
+    cur = ref(k0);
 }
 
 __device__ __forceinline__ void tri_test(const DevScene &sc, uint32_t i, float4 a, float4 b, float4 c, V3 o,
@@ -591,7 +660,7 @@ __device__ __forceinline__ void trace_persistent(const DevScene &sc, const IO &i
                 if (kSpheres) sbest = best;
                 sphere_phase = kSpheres && sc.sroot != kNone;
                 t_cur = t_q;
-                cur = enter_root(sc, sphere_phase, o, inv, t_min, t_cur);
+                cur = enter_root<kCull>(sc, sphere_phase, o, inv, t_min, t_cur);
                 sp = 0;
                 leaf = kNone;
             }
@@ -614,9 +683,13 @@ __device__ __forceinline__ void trace_persistent(const DevScene &sc, const IO &i
             const bool in = slot != kNone && cur != kNone && !(cur & kLeafBit);
             if (__ballot(in && leaf == kNone) == 0) break;
             if (in) {
-                const float4 *nodes = (kSpheres && sphere_phase) ? sc.snodes : sc.mnodes;
-                node_step<kCull>(nodes, o, inv, t_min, t_cur, best.t, cur, sp, st, cache,
-                                 (kSpheres && sphere_phase) ? 0u : ncache);
+                if (kCull && !(kSpheres && sphere_phase)) {
+                    node_step_wide(sc.mwide, o, inv, t_min, t_cur, best.t, cur, sp, st);
+                } else {
+                    const float4 *nodes = (kSpheres && sphere_phase) ? sc.snodes : sc.mnodes;
+                    node_step<kCull>(nodes, o, inv, t_min, t_cur, best.t, cur, sp, st, cache,
+                                     (kSpheres && sphere_phase) ? 0u : ncache);
+                }
             }
         }
         // ---- leaves ----------------------------------------------------------
@@ -641,7 +714,7 @@ __device__ __forceinline__ void trace_persistent(const DevScene &sc, const IO &i
                     best.prim = kNone;
                     best.id = kNone;
                     best.t = t_cur;
-                    cur = enter_root(sc, false, o, inv, t_min, t_cur);
+                    cur = enter_root<kCull>(sc, false, o, inv, t_min, t_cur);
                 }
             }
             if (!sphere_phase && cur == kNone) {

@@ -49,61 +49,90 @@ __host__ __device__ inline float qnode_plane(uint32_t w, int byte, float step, f
     return fmaf((float)((w >> (8 * byte)) & 0xFFu), step, origin);
 }
 
+// Grid of one axis over [mn, mx]: the word (origin bits with the step's
+// biased exponent in the low byte), or false where a plane is not finite or
+// 255 steps overflow FP32.
+inline bool qgrid_axis(float mn, float mx, uint32_t &word) {
+    if (!std::isfinite(mn) || !std::isfinite(mx)) return false;
+    // Origin: mn with its low byte cleared, moved down one unit of 256 ulps
+    // where clearing raised it (a negative mn).
+    uint32_t bits;
+    memcpy(&bits, &mn, 4);
+    uint32_t ob = bits & 0xFFFFFF00u;
+    float org;
+    memcpy(&org, &ob, 4);
+    if (org > mn) {
+        ob = ((bits | 0xFFu) + 1u) & 0xFFFFFF00u;
+        memcpy(&org, &ob, 4);
+    }
+    if (!std::isfinite(org) || org > mn) return false;
+    // Step: the smallest power of two whose 255th grid plane reaches mx.
+    uint32_t e = 1;
+    for (; e <= 254u; ++e) {
+        const float top = qnode_plane(255u, 0, qnode_step(e), org);
+        if (std::isfinite(top) && top >= mx) break;
+    }
+    if (e > 254u) return false;
+    word = ob | e;
+    return true;
+}
+
+// The plane byte of a lower (upper) plane v on an axis grid: the largest q
+// whose decoded value is <= v (the smallest q whose value is >= v).  The
+// caller checks that the decoded plane lies on the outer side of v.
+inline uint32_t qgrid_lower(uint32_t word, float v) {
+    const float st = qnode_step(word), o = qnode_origin(word);
+    auto dec = [&](uint32_t q) { return qnode_plane(q, 0, st, o); };
+    double g = std::floor(((double)v - (double)o) / (double)st);
+    uint32_t q = (uint32_t)std::min(255.0, std::max(0.0, g));
+    while (q > 0u && dec(q) > v) --q;
+    while (q < 255u && dec(q + 1u) <= v) ++q;
+    return q;
+}
+
+inline uint32_t qgrid_upper(uint32_t word, float v) {
+    const float st = qnode_step(word), o = qnode_origin(word);
+    auto dec = [&](uint32_t q) { return qnode_plane(q, 0, st, o); };
+    double g = std::ceil(((double)v - (double)o) / (double)st);
+    uint32_t q = (uint32_t)std::min(255.0, std::max(0.0, g));
+    while (q < 255u && dec(q) < v) ++q;
+    while (q > 0u && dec(q - 1u) >= v) --q;
+    return q;
+}
+
+// Plane bytes of n boxes on one shared grid, 6 per box in the order
+// (lo.x, hi.x, lo.y, hi.y, lo.z, hi.z), and the three grid words.  False
+// where the grid cannot express a box.
+inline bool qgrid_encode(const float (*lo)[3], const float (*hi)[3], int n, uint32_t grid[3], uint32_t *bytes) {
+    for (int k = 0; k < 3; ++k) {
+        float mn = lo[0][k], mx = hi[0][k];
+        for (int c = 1; c < n; ++c) {
+            mn = std::min(mn, lo[c][k]);
+            mx = std::max(mx, hi[c][k]);
+        }
+        for (int c = 0; c < n; ++c)
+            if (!std::isfinite(lo[c][k]) || !std::isfinite(hi[c][k])) return false;
+        if (!qgrid_axis(mn, mx, grid[k])) return false;
+        const float st = qnode_step(grid[k]), o = qnode_origin(grid[k]);
+        for (int c = 0; c < n; ++c) {
+            const uint32_t ql = qgrid_lower(grid[k], lo[c][k]), qh = qgrid_upper(grid[k], hi[c][k]);
+            if (qnode_plane(ql, 0, st, o) > lo[c][k] || qnode_plane(qh, 0, st, o) < hi[c][k]) return false;
+            bytes[6 * c + 2 * k] = ql;
+            bytes[6 * c + 2 * k + 1] = qh;
+        }
+    }
+    return true;
+}
+
 // Encodes the child pair (lo0, hi0), (lo1, hi1) with references ref0, ref1.
 // Returns false where the grid cannot express a box (a non-finite plane, or
 // a range whose 255 steps overflow FP32); w is then unspecified.
 inline bool qnode_encode(const float lo0[3], const float hi0[3], const float lo1[3], const float hi1[3],
                          uint32_t ref0, uint32_t ref1, uint32_t w[kQuantWords]) {
+    const float lo[2][3] = {{lo0[0], lo0[1], lo0[2]}, {lo1[0], lo1[1], lo1[2]}};
+    const float hi[2][3] = {{hi0[0], hi0[1], hi0[2]}, {hi1[0], hi1[1], hi1[2]}};
     uint32_t bytes[12];
-    for (int k = 0; k < 3; ++k) {
-        if (!std::isfinite(lo0[k]) || !std::isfinite(hi0[k]) || !std::isfinite(lo1[k]) || !std::isfinite(hi1[k]))
-            return false;
-        const float mn = std::min(lo0[k], lo1[k]), mx = std::max(hi0[k], hi1[k]);
-        // Origin: mn with its low byte cleared, moved down one unit of 256
-        // ulps where clearing raised it (a negative mn).
-        uint32_t bits;
-        memcpy(&bits, &mn, 4);
-        uint32_t ob = bits & 0xFFFFFF00u;
-        float org;
-        memcpy(&org, &ob, 4);
-        if (org > mn) {
-            ob = ((bits | 0xFFu) + 1u) & 0xFFFFFF00u;
-            memcpy(&org, &ob, 4);
-        }
-        if (!std::isfinite(org) || org > mn) return false;
-        // Step: the smallest power of two whose 255th grid plane reaches mx.
-        uint32_t e = 1;
-        for (; e <= 254u; ++e) {
-            const float top = qnode_plane(255u, 0, qnode_step(e), org);
-            if (std::isfinite(top) && top >= mx) break;
-        }
-        if (e > 254u) return false;
-        const uint32_t word = ob | e;
-        const float st = qnode_step(word), o = qnode_origin(word);
-        auto dec = [&](uint32_t q) { return qnode_plane(q, 0, st, o); };
-        auto lower = [&](float v) {  // largest q with dec(q) <= v
-            double g = std::floor(((double)v - (double)o) / (double)st);
-            uint32_t q = (uint32_t)std::min(255.0, std::max(0.0, g));
-            while (q > 0u && dec(q) > v) --q;
-            while (q < 255u && dec(q + 1u) <= v) ++q;
-            return q;
-        };
-        auto upper = [&](float v) {  // smallest q with dec(q) >= v
-            double g = std::ceil(((double)v - (double)o) / (double)st);
-            uint32_t q = (uint32_t)std::min(255.0, std::max(0.0, g));
-            while (q < 255u && dec(q) < v) ++q;
-            while (q > 0u && dec(q - 1u) >= v) --q;
-            return q;
-        };
-        bytes[2 * k] = lower(lo0[k]);
-        bytes[2 * k + 1] = upper(hi0[k]);
-        bytes[6 + 2 * k] = lower(lo1[k]);
-        bytes[7 + 2 * k] = upper(hi1[k]);
-        if (dec(bytes[2 * k]) > lo0[k] || dec(bytes[2 * k + 1]) < hi0[k] || dec(bytes[6 + 2 * k]) > lo1[k] ||
-            dec(bytes[7 + 2 * k]) < hi1[k])
-            return false;
-        w[k] = word;
-    }
+    if (!qgrid_encode(lo, hi, 2, w, bytes)) return false;
     for (int j = 0; j < 3; ++j)
         w[3 + j] = bytes[4 * j] | (bytes[4 * j + 1] << 8) | (bytes[4 * j + 2] << 16) | (bytes[4 * j + 3] << 24);
     w[6] = ref0;
