@@ -87,14 +87,98 @@ inline float wide_half_area(const float lo[3], const float hi[3]) {
     return x * y + y * z + z * x;
 }
 
-// Groups the binary tree under the internal root into wide nodes.  Indices
-// are given depth-first: visiting a node gives its interior children the
-// next consecutive indices, and its first child's subtree is visited before
-// the second's, so a path down the tree reads nearby nodes.  False where a
-// reference points outside the pairs.
+// Groups the binary tree under the internal root into wide nodes.  Which
+// binary nodes become wide nodes is chosen to minimise the summed surface
+// area of the wide nodes (the expected number of wide-node visits of a ray
+// under the surface-area heuristic), each holding at most 4 children, by a
+// dynamic program over the binary tree (the collapse of Ylitie, Karras and
+// Laine, "Efficient incoherent ray traversal on GPUs through compressed wide
+// BVHs", HPG 2017, with leaves kept as they are):
+//   f(s, k)  least area of the wide nodes below child slot s when s's subtree
+//            takes at most k child slots of its wide parent: 0 for a leaf;
+//            for an internal node, area(s) + g(s, 4) as a wide node of its
+//            own (one slot), or g(s, k) with its children lifted (k >= 2);
+//   g(s, m)  min over a + b = m of f(left, a) + f(right, b).
+// Indices are given depth-first: visiting a node gives its interior children
+// the next consecutive indices, and its first child's subtree is visited
+// before the second's, so a path down the tree reads nearby nodes.  False
+// where a reference points outside the pairs or the tree is not a tree.
 inline bool wide_plan(const WidePairs &bp, uint32_t root, WidePlan &out) {
     out = WidePlan{};
     if (!bp.interior(root) || root >= bp.n_pairs) return false;
+    // Post-order over the internal nodes reachable from the root.
+    const size_t np = bp.n_pairs;
+    std::vector<uint8_t> seen(np, 0);
+    std::vector<uint32_t> post;
+    {
+        std::vector<std::pair<uint32_t, bool>> st{{root, false}};
+        while (!st.empty()) {
+            auto [p, done] = st.back();
+            st.pop_back();
+            if (done) {
+                post.push_back(p);
+                continue;
+            }
+            if (seen[p]) return false;
+            seen[p] = 1;
+            st.push_back({p, true});
+            for (uint32_t side = 0; side < 2; ++side) {
+                const uint32_t r = bp.ref(2u * p + side);
+                if (!bp.interior(r)) continue;
+                if (r >= np) return false;
+                st.push_back({r, false});
+            }
+        }
+    }
+    // f per child slot (k = 1..4), g and its split per pair (m = 2..4),
+    // children before parents.
+    std::vector<float> f(2 * np * 4, 0.0f), g(np * 3, 0.0f);
+    std::vector<uint8_t> split(np * 3, 1);
+    auto F = [&](uint32_t slot, uint32_t k) -> float & { return f[(size_t)slot * 4 + (k - 1)]; };
+    auto G = [&](uint32_t p, uint32_t m) -> float & { return g[(size_t)p * 3 + (m - 2)]; };
+    for (const uint32_t p : post) {
+        for (uint32_t side = 0; side < 2; ++side) {
+            const uint32_t slot = 2u * p + side, r = bp.ref(slot);
+            if (!bp.interior(r)) continue;  // a leaf: f = 0
+            float lo[3], hi[3];
+            bp.box(slot, lo, hi);
+            const float own = std::max(wide_half_area(lo, hi), 0.0f) + G(r, 4);
+            F(slot, 1) = own;
+            for (uint32_t k = 2; k <= 4; ++k) F(slot, k) = std::min(own, G(r, k));
+        }
+        for (uint32_t m = 2; m <= 4; ++m) {
+            uint32_t arg = 1;
+            float best = F(2u * p, 1) + F(2u * p + 1u, m - 1);
+            for (uint32_t a = 2; a < m; ++a) {
+                const float c = F(2u * p, a) + F(2u * p + 1u, m - a);
+                if (c < best) {
+                    best = c;
+                    arg = a;
+                }
+            }
+            G(p, m) = best;
+            split[(size_t)p * 3 + (m - 2)] = (uint8_t)arg;
+        }
+    }
+    // The child slots of a wide node over pair p with m slots to fill: a
+    // child slot stays a slot (a leaf, or a wide node of its own) or is
+    // replaced by its children's slots, as the program chose.
+    auto gather = [&](uint32_t p, uint32_t m, WidePlan::Node &nd) {
+        std::vector<std::pair<uint32_t, uint32_t>> todo{{p, m}};  // (pair, slots)
+        while (!todo.empty()) {
+            auto [q, k] = todo.back();
+            todo.pop_back();
+            const uint32_t a = split[(size_t)q * 3 + (k - 2)];
+            for (uint32_t side = 0; side < 2; ++side) {
+                const uint32_t slot = 2u * q + side, kk = side ? k - a : a;
+                const uint32_t r = bp.ref(slot);
+                if (!bp.interior(r) || kk == 1 || F(slot, 1) <= G(r, kk))
+                    nd.slot[nd.n++] = slot;
+                else
+                    todo.push_back({r, kk});
+            }
+        }
+    };
     std::vector<uint32_t> need{0u};  // stack entries held on entering each node
     out.src.push_back(root);
     out.nodes.resize(1);
@@ -105,29 +189,7 @@ inline bool wide_plan(const WidePairs &bp, uint32_t root, WidePlan &out) {
         todo.pop_back();
         out.visit.push_back(w);
         WidePlan::Node nd;
-        nd.n = 2;
-        nd.slot[0] = 2u * out.src[w];
-        nd.slot[1] = 2u * out.src[w] + 1u;
-        while (nd.n < 4u) {
-            int pick = -1;
-            float best = -1.0f;
-            for (uint32_t k = 0; k < nd.n; ++k) {
-                const uint32_t r = bp.ref(nd.slot[k]);
-                if (!bp.interior(r)) continue;
-                if (r >= bp.n_pairs) return false;
-                float lo[3], hi[3];
-                bp.box(nd.slot[k], lo, hi);
-                const float a = wide_half_area(lo, hi);
-                if (pick < 0 || a > best) {
-                    pick = (int)k;
-                    best = a;
-                }
-            }
-            if (pick < 0) break;
-            const uint32_t r = bp.ref(nd.slot[pick]);
-            nd.slot[pick] = 2u * r;
-            nd.slot[nd.n++] = 2u * r + 1u;
-        }
+        gather(out.src[w], 4, nd);
         const uint32_t held = need[w] + nd.n - 1u;  // a node step pushes all but the nearest hit child
         out.stack_need = std::max(out.stack_need, held);
         const uint32_t first = (uint32_t)out.src.size();
@@ -135,7 +197,6 @@ inline bool wide_plan(const WidePairs &bp, uint32_t root, WidePlan &out) {
         for (uint32_t k = 0; k < nd.n; ++k) {
             const uint32_t r = bp.ref(nd.slot[k]);
             if (!bp.interior(r)) continue;
-            if (r >= bp.n_pairs) return false;
             out.src.push_back(r);
             need.push_back(held);
         }
