@@ -74,6 +74,20 @@ inline bool qgrid_axis(float mn, float mx, uint32_t &word) {
     }
     if (e > 254u) return false;
     word = ob | e;
+    // Where the word format allows it, a multiple of the step as origin
+    // instead (the same or the next step): then every plane on the step grid
+    // (voxel coordinates, for one) decodes exactly and adds no thickness.
+    for (uint32_t ea = e; ea <= std::min<uint32_t>(e + 1u, 254u); ++ea) {
+        const double st = (double)qnode_step(ea);
+        const float a = (float)(std::floor((double)mn / st) * st);
+        uint32_t ab;
+        memcpy(&ab, &a, 4);
+        if ((ab & 0xFFu) != 0u || !std::isfinite(a) || a > mn) continue;
+        const float top = qnode_plane(255u, 0, qnode_step(ea), a);
+        if (!std::isfinite(top) || top < mx) continue;
+        word = ab | ea;
+        break;
+    }
     return true;
 }
 
