@@ -440,6 +440,7 @@ def _main(argv=None):
     t.add_argument("--warm-steps", type=int, default=2000, help="log-L2 steps before --loss")
     t.add_argument("--nee", action="store_true", help="render inputs and references with next-event estimation")
     t.add_argument("--out", default=None, help="default: the package's weights file for the estimator")
+    t.add_argument("--init", default=None, help="continue training from these weights (their .json beside them)")
     e = sub.add_parser("eval")
     e.add_argument("--weights", default=WEIGHTS_PATH)
     e.add_argument("--weights-nee", default=WEIGHTS_NEE_PATH)
@@ -461,6 +462,13 @@ def _main(argv=None):
             t_pairs = time.time() - t0
             log(f"pairs rendered in {t_pairs:.1f} s")
             dn = Denoiser(ctx, seed=a.seed)
+            init_meta = None
+            if a.init:
+                dn.set_weights(np.load(a.init, allow_pickle=False))
+                side = os.path.splitext(a.init)[0] + ".json"
+                if os.path.exists(side):
+                    with open(side) as f:
+                        init_meta = json.load(f)
             t1 = time.time()
             losses = train(dn, pairs, a.steps, batch=a.batch, lr=a.lr, seed=a.seed, log=log, loss=a.loss,
                            warm_steps=a.warm_steps)
@@ -471,7 +479,8 @@ def _main(argv=None):
                                    "spp_ref": a.spp_ref, "nee": a.nee, "steps": a.steps, "batch": a.batch,
                                    "lr": a.lr, "seed": a.seed, "loss": a.loss, "warm_steps": a.warm_steps,
                                    "loss_first": losses[0], "loss_last": losses[-1],
-                                   "pairs_s": round(t_pairs, 1), "train_s": round(t_train, 1)})
+                                   "pairs_s": round(t_pairs, 1), "train_s": round(t_train, 1),
+                                   **({"init": os.path.basename(a.init), "init_meta": init_meta} if a.init else {})})
             log(f"weights -> {out}")
         elif a.cmd == "infer-bench":
             dn = load_trained(ctx)
