@@ -483,10 +483,11 @@ def roofline(ctx, w, step, segments, counters, peaks):
         res["counters"] = kf
         res["counters_source"] = ("rocprofv3 --pmc passes of `bench.py --probe` in this invocation: "
                                   + " | ".join(" ".join(p) for p in PMC_PASSES))
-        res["binding"] = ("not HBM bandwidth: the loads each node step issues through the vector-memory path "
-                          "(counters.ta_busy_frac) and the latency of dependent node fetches over lanes that "
-                          "diverge (counters.lane_efficiency); added to every node step, one more 16-B load "
-                          "costs the launch 8.5%, while 24 more VALU instructions cost 1.4%")
+        res["binding"] = ("not HBM bandwidth: VALU issue over lanes that diverge (counters.valu_inst_rate_vs_fma_probe, "
+                          "counters.lane_efficiency) and the latency of dependent node fetches; each 4-wide node "
+                          "step slab-tests 4 boxes exactly (about 150 VALU instructions) from three 16-B loads, and "
+                          "the vector-memory address unit (counters.ta_busy_frac) is no longer the busiest unit, "
+                          "as it was on the binary pairs, where one more 16-B load per node step cost 8.5%")
     return res
 
 
