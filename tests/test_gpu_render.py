@@ -193,6 +193,19 @@ def test_voxel_terrain_c3_small(ctx, c3_small):
     np.testing.assert_allclose(den[..., :3], oden[..., :3], rtol=2e-5, atol=1e-5)
 
 
+def test_voxel_terrain_with_spheres(ctx, c3_spheres):
+    """Spheres in the voxel terrain: the culling-stack extension kernel with a
+    sphere phase (quantized sphere pairs) before the mesh phase (4-wide
+    nodes), every bounce (no first-bounce packets with spheres)."""
+    w, sb = c3_spheres
+    ctx.upload_scene(sb)
+    acc, gb = ctx.render(w.camera, w.width, w.height, w.spp, 0, w.depth)
+    ref, rgb = oracle.render(oracle.Scene(sb), w.camera, w.width, w.height, w.spp, 0, w.depth)
+    same_g, ok, r = _parity(acc, gb, ref, rgb)
+    print(f"c3 + spheres: gbuf {same_g:.4f} px {ok:.4f} psnr {r.psnr:.1f}")
+    assert same_g >= 0.999 and ok >= 0.98, (same_g, ok)
+
+
 def test_voxel_terrain_pixel_major_queue(ctx):
     """Paths are queued pixel-major (a pixel's frames adjacent): at an odd
     size with 3 frames per batch every (pixel, frame) path must still be

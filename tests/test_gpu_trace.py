@@ -169,3 +169,22 @@ def test_c3_large_scene_bit_exact(ctx):
     gid, gt, oid, ot, bad = _cmp(ctx, w.buffers, o, d)
     assert len(bad) == 0, f"{len(bad)} mismatches, first {bad[:5]}: gpu {gid[bad[:5]]} {gt[bad[:5]]} oracle {oid[bad[:5]]} {ot[bad[:5]]}"
     assert np.mean(gid != T.ID_MISS) > 0.1
+
+
+def test_c3_with_spheres_bit_exact(ctx, c3_spheres):
+    """Spheres and the 2.16M-triangle terrain: ids and t equal the oracle's
+    bit for bit, for incoherent and axis-parallel rays."""
+    w, sb = c3_spheres
+    lo, hi = configs.scene_aabb(sb)
+    o, d = scenes.microbench_rays(1 << 17, lo, hi, seed=21)
+    rng = np.random.default_rng(8)
+    n_ax = 1 << 13
+    oa = (lo + rng.random((n_ax, 3)) * (hi - lo)).astype(np.float32)
+    axes = np.eye(3, dtype=np.float32)[rng.integers(0, 3, n_ax)] * rng.choice([-1.0, 1.0], (n_ax, 1)).astype(np.float32)
+    o = np.concatenate([o, oa])
+    d = np.concatenate([d, axes])
+    print("trace c3 + 300 spheres", flush=True)
+    gid, gt, oid, ot, bad = _cmp(ctx, sb, o, d)
+    assert len(bad) == 0, f"{len(bad)} mismatches, first {bad[:5]}: gpu {gid[bad[:5]]} {gt[bad[:5]]} oracle {oid[bad[:5]]} {ot[bad[:5]]}"
+    assert np.mean(gid != T.ID_MISS) > 0.1
+    assert np.sum((oid != T.ID_MISS) & ((oid & T.ID_SPHERE_BIT) != 0)) > 100  # spheres are hit too
