@@ -123,6 +123,27 @@ def test_gpu_primal_matches_oracle_fp64(ctx, mid):
 
 
 @pytest.mark.gpu
+def test_gpu_primal_on_a_scene_that_spills_the_l2(ctx):
+    """The 2.16M-triangle voxel terrain, grass and stone as parameters: the
+    adjoint's per-lane traversal then reads quantized child pairs with the
+    culling stack.  Bar as above: 99% of pixels within 1e-4 relative (+1e-5)
+    of the FP64 oracle."""
+    from mcrt_amd import configs
+
+    w = configs.c3(width=48, height=27, spp=2, depth=4)
+    pmap = np.full(64, -1, np.int32)
+    pmap[w.loaded.name_to_idx["grass"]] = 0
+    pmap[w.loaded.name_to_idx["stone"]] = 1
+    p = np.array([[0.6, 0.3, 0.2, 0.0], [0.2, 0.5, 0.7, 0.1]], np.float32)
+    ctx.upload_scene(w.buffers)
+    print("diff render c3 48x27 2spp depth 4", flush=True)
+    g = ctx.diff_render(w.camera, w.width, w.height, w.spp, 0, w.depth, p, pmap)
+    o = oracle.diff_render64(oracle.Scene(w.buffers), w.camera, w.width, w.height, w.spp, 0, w.depth, p, pmap)
+    ok = np.all(np.abs(g[..., :3] - o) <= 1e-4 * np.abs(o) + 1e-5, axis=-1)
+    assert ok.mean() >= 0.99, ok.mean()
+
+
+@pytest.mark.gpu
 def test_gpu_path_replay_gradient_matches_oracle_and_fd(ctx, mid):
     """Bar: the gradient vector within 1e-3 (relative L2) of the oracle's
     forward-mode gradient and of FP64 central differences."""
